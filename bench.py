@@ -1,0 +1,173 @@
+"""Benchmark: SH_to_spat + spat_to_SH transform pairs per second on one MI355X.
+
+Workload (one step): synthesis then analysis of a batch of 16 independent FP64 fields at
+lmax = mmax = 1023 on the Gauss grid nlat = 1536, nphi = 3072 (theta-contiguous), with
+a_lm ~ N(0, (l+1)^-2) per component (SplitMix64/Box-Muller, seed 1 + field), inputs
+resident in HBM.  value = 16 * steps / elapsed  (pairs/s, whole job).
+
+roofline: the dominant kernel (largest share of the timed region) timed live with HIP
+events recorded by the library on its compute stream; achieved = algorithmic FLOPs per
+launch (2*nlat*NLM per field for a Legendre kernel, 2.5*nlat*nphi*log2(nphi) per field for
+an FFT kernel) / average launch duration, against the FP64 peak 78.6 TFLOP/s.
+cpu_baseline: the CPU oracle (oracle/, an OpenMP FP64 restatement of SHTns's transform)
+timed on this host's cores on one-field pairs of the same workload.
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+LMAX, NLAT, NPHI, BATCH, SEED = 1023, 1536, 3072, 16, 1
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+
+
+def nlm_of(lmax):
+    return (lmax + 1) * (lmax + 2) // 2
+
+
+def leg_flops(lmax, nlat):
+    return 2.0 * nlat * nlm_of(lmax)
+
+
+def fft_flops(nlat, nphi):
+    return 2.5 * nlat * nphi * math.log2(nphi)
+
+
+def cpu_baseline(budget_s=10.0):
+    """Time the CPU oracle on one-field SH_to_spat + spat_to_SH pairs of the workload."""
+    from oracle.oracle import Oracle, set_threads
+    from shtns_amd.synthetic import spectral_field
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    set_threads(cores)
+    o = Oracle(LMAX, LMAX, 1, NLAT, NPHI)
+    q = spectral_field(LMAX, LMAX, seed=SEED)
+    v = o.synth(q)          # warm-up pair (not timed)
+    o.analys(v)
+    n, t0 = 0, time.perf_counter()
+    while True:
+        v = o.synth(q)
+        o.analys(v)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or n >= 50:
+            break
+    return {"value": n / el, "unit": "pairs/s", "cores": cores, "kind": "port",
+            "sample": f"{n} one-field pairs (lmax=1023, 1536x3072, FP64) of the same workload, "
+                      f"{el:.1f} s, OpenMP over {cores} threads"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is described in DESIGN.md but not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import shtns_amd
+    from shtns_amd import _lib
+    from shtns_amd.synthetic import spectral_batch
+
+    lib = _lib.load()
+    dev = torch.device("cuda:0")
+    s = shtns_amd.sht(LMAX)
+    s.set_grid(NLAT, NPHI, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+    nlm, nspat = s.nlm, s.nspat
+    q = torch.from_numpy(spectral_batch(LMAX, LMAX, BATCH, seed=SEED)).to(dev)
+    v = torch.empty((BATCH, NPHI, NLAT), dtype=torch.float64, device=dev)
+    q2 = torch.empty_like(q)
+    stream = torch.cuda.current_stream(dev)
+    lib.cushtns_set_streams(s._cfg, ctypes.c_void_p(stream.cuda_stream), ctypes.c_void_p(stream.cuda_stream))
+    qp, vp, q2p = (ctypes.c_void_p(t.data_ptr()) for t in (q, v, q2))
+
+    def step():
+        lib.cu_SH_to_spat_batch(s._cfg, qp, vp, LMAX, BATCH, nlm, nspat)
+        lib.cu_spat_to_SH_batch(s._cfg, vp, q2p, LMAX, BATCH, nspat, nlm)
+
+    for _ in range(args.warmup):
+        step()
+    _lib.check_error("warmup")
+    torch.cuda.synchronize()
+    lib.shtns_profile(s._cfg, 1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.check_error("timed loop")
+    ms = (ctypes.c_double * 8)()
+    cnt = (ctypes.c_int * 8)()
+    lib.shtns_profile_get(s._cfg, ms, cnt)
+    lib.shtns_profile(s._cfg, 0)
+    elapsed = t1 - t0
+    pairs = BATCH * args.steps
+    value = pairs / elapsed
+
+    # accuracy of the timed output (round trip), outside the timed region
+    err = float(torch.linalg.norm((q2 - q).flatten()) / torch.linalg.norm(q.flatten()))
+
+    names = ["legendre_synth", "legendre_anal", "fft_synth", "fft_anal"]
+    flops = [leg_flops(LMAX, NLAT) * BATCH, leg_flops(LMAX, NLAT) * BATCH,
+             fft_flops(NLAT, NPHI) * BATCH, fft_flops(NLAT, NPHI) * BATCH]
+    kern = {}
+    for k, nm in enumerate(names):
+        if cnt[k]:
+            avg = ms[k] / cnt[k] * 1e-3
+            kern[nm] = {"avg_ms": avg * 1e3, "launches": cnt[k], "share": ms[k] * 1e-3 / elapsed,
+                        "tflops": flops[k] / avg / 1e12}
+    dom = max(kern, key=lambda n: kern[n]["share"])
+    dk = kern[dom]
+    roof = {"bound": "mfma", "achieved": round(dk["tflops"], 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(dk["tflops"] / FP64_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom}
+    alg_total = (leg_flops(LMAX, NLAT) + fft_flops(NLAT, NPHI)) * 2 * pairs
+    out = {
+        "metric": "SH_to_spat+spat_to_SH pairs/s at lmax=1023 (FP64)",
+        "value": round(value, 2),
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (SplitMix64/Box-Muller a_lm ~ N(0,(l+1)^-2), seed 1+field)",
+        "config": {"workload": "SH_to_spat+spat_to_SH, lmax=mmax=1023, Gauss nlat=1536 nphi=3072, "
+                               "theta-contiguous, batch 16 FP64 fields",
+                   "lmax": LMAX, "nlat": NLAT, "nphi": NPHI, "batch": BATCH, "parallelism": "single GPU"},
+        "fp64_frac_of_peak_whole_pair": round(alg_total / elapsed / 1e12 / FP64_PEAK_TFLOPS, 4),
+        "round_trip_rel_l2": err,
+        "kernels": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv for kk, vv in d.items()}
+                    for k, d in kern.items()},
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(args.cpu_budget)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,139 @@
+/*
+ * shtns.h -- public C API of shtns_amd, a spherical harmonic transform engine for one
+ * AMD MI355X (gfx950).  It exports the same names, signatures, flag values and public
+ * struct prefix as SHTns 3.5.2's shtns.h so existing callers link against it unchanged.
+ * Each declaration names the SHTns interface it stands in for (file:line of shtns.h
+ * in SHTns 3.5.2).
+ *
+ * Scope: Gauss grids, the scalar transforms SH_to_spat/spat_to_SH and the 2D vector
+ * transforms SHsphtor_to_spat/spat_to_SHsphtor (plus their _l truncated forms), computed
+ * on the GPU by hand-written HIP kernels.  Host-pointer transforms copy to the device,
+ * run there and copy back; the device-pointer variants are in shtns_hip.h.
+ */
+#ifndef SHTNS_AMD_SHTNS_H
+#define SHTNS_AMD_SHTNS_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+#include <complex>
+typedef std::complex<double> cplx;   /* shtns.h:22-30: 2 doubles, re then im */
+typedef std::complex<float> cplx_f;
+extern "C" {
+#else
+#include <complex.h>
+typedef complex double cplx;
+typedef complex float cplx_f;
+#endif
+
+#define SHTNS_INTERFACE 0x30500                 /* shtns.h:38 */
+
+typedef struct shtns_info *shtns_cfg;           /* shtns.h:41 */
+
+/* normalizations (shtns.h:48-55).  Only sht_orthonormal is implemented here; the
+   others are accepted by shtns_create and rejected with an error message. */
+enum shtns_norm { sht_orthonormal, sht_fourpi, sht_schmidt, sht_for_rotations };
+#define SHT_NO_CS_PHASE (256 * 4)
+#define SHT_REAL_NORM (256 * 8)
+
+/* grid / algorithm selectors (shtns.h:58-75).  Gauss grids are implemented:
+   sht_gauss, sht_auto, sht_quick_init and sht_gauss_fly all select the Gauss grid. */
+enum shtns_type { sht_gauss, sht_auto, sht_reg_fast, sht_reg_dct, sht_quick_init, sht_reg_poles, sht_gauss_fly };
+#define SHT_NATIVE_LAYOUT 0
+#define SHT_THETA_CONTIGUOUS 256
+#define SHT_PHI_CONTIGUOUS (256 * 2)
+#define SHT_SOUTH_POLE_FIRST (256 * 32)
+#define SHT_SCALAR_ONLY (256 * 16)
+#define SHT_LOAD_SAVE_CFG (256 * 64)
+#define SHT_ALLOW_GPU (256 * 128)
+#define SHT_ALLOW_PADDING (256 * 256)
+
+#ifndef SHTNS_PRIVATE
+/* Read-only public prefix of the configuration (shtns.h:79-94), same member order and types. */
+struct shtns_info {
+    const unsigned int nlm;
+    const unsigned short lmax;
+    const unsigned short mmax;
+    const unsigned short mres;
+    const unsigned short nlat_2;
+    const unsigned int nlat;
+    const unsigned int nphi;
+    const unsigned int nspat;
+    const unsigned short *const li;
+    const unsigned short *const mi;
+    const double *const ct;
+    const double *const st;
+    const unsigned int nlat_padded;
+    const unsigned int nlm_cplx;
+};
+#endif
+
+/* index macros (shtns.h:104-106, 120) */
+#define LiM(shtns, l, im) ((((im) * (2 * shtns->lmax + 2 - ((im) + 1) * shtns->mres)) >> 1) + (l))
+#define LM(shtns, l, m) (((((unsigned short)(m)) / shtns->mres) * (2 * shtns->lmax + 2 - ((m) + shtns->mres))) >> 1) + (l))
+#define NSPAT_ALLOC(shtns) (shtns->nspat)
+#define PHI_DEG(shtns, ip) (360. / (shtns->nphi * shtns->mres)) * (ip)
+#define PHI_RAD(shtns, ip) (2. * M_PI / (shtns->nphi * shtns->mres)) * (ip)
+
+/* sizes and information (shtns.h:131-142, 146) */
+long nlm_calc(long lmax, long mmax, long mres);                 /* shtns.h:131 */
+long nlm_cplx_calc(long lmax, long mmax, long mres);            /* shtns.h:134 */
+void shtns_verbose(int);                                        /* shtns.h:136 */
+void shtns_print_version(void);                                 /* shtns.h:137 */
+const char *shtns_get_build_info(void);                         /* shtns.h:138 */
+void shtns_print_cfg(shtns_cfg);                                /* shtns.h:142 */
+
+/* initialization (shtns.h:147-177) */
+shtns_cfg shtns_init(enum shtns_type flags, int lmax, int mmax, int mres, int nlat, int nphi);   /* :149 */
+shtns_cfg shtns_create(int lmax, int mmax, int mres, enum shtns_norm norm);                     /* :151 */
+int shtns_set_grid(shtns_cfg, enum shtns_type flags, double eps, int nlat, int nphi);           /* :153 */
+int shtns_set_grid_auto(shtns_cfg, enum shtns_type flags, double eps, int nl_order, int *nlat, int *nphi); /* :155 */
+int shtns_use_threads(int num_threads);                         /* :159 */
+int shtns_use_gpu(int device_id);                               /* :161 */
+void shtns_reset(void);                                         /* :163 */
+void shtns_destroy(shtns_cfg);                                  /* :164 */
+void shtns_unset_grid(shtns_cfg);                               /* :165 */
+void *shtns_malloc(size_t bytes);                               /* :170 */
+void shtns_free(void *p);                                       /* :171 */
+
+/* special values (shtns.h:178-186) */
+double sh00_1(shtns_cfg);                                       /* :180 */
+double sh10_ct(shtns_cfg);                                      /* :181 */
+double sh11_st(shtns_cfg);                                      /* :182 */
+double shlm_e1(shtns_cfg, int l, int m);                        /* :183 */
+int shtns_gauss_wts(shtns_cfg, double *wts);                    /* :185 */
+
+/* Legendre functions (shtns.h:227-228) */
+int legendre_sphPlm_array(shtns_cfg shtns, const int lmax, const int im, const double x, double *yl);
+
+/* scalar transforms (shtns.h:262-267).  spat_to_SH leaves Vr unchanged here
+   (SHTns only promises nothing about it). */
+void spat_to_SH(shtns_cfg shtns, double *Vr, cplx *Qlm);
+void SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr);
+
+/* 2D vector transforms (shtns.h:283-289) */
+void spat_to_SHsphtor(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm);
+void SHsphtor_to_spat(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp);
+void SHsph_to_spat(shtns_cfg, cplx *Slm, double *Vt, double *Vp);
+void SHtor_to_spat(shtns_cfg, cplx *Tlm, double *Vt, double *Vp);
+
+/* truncated transforms at degree ltr <= lmax (shtns.h:315-324) */
+void spat_to_SH_l(shtns_cfg, double *Vr, cplx *Qlm, int ltr);
+void SH_to_spat_l(shtns_cfg, cplx *Qlm, double *Vr, int ltr);
+void SHsphtor_to_spat_l(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp, int ltr);
+void SHsph_to_spat_l(shtns_cfg, cplx *Slm, double *Vt, double *Vp, int ltr);
+void SHtor_to_spat_l(shtns_cfg, cplx *Tlm, double *Vt, double *Vp, int ltr);
+void spat_to_SHsphtor_l(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr);
+
+/* point evaluation (shtns.h:350) */
+double SH_to_point(shtns_cfg, cplx *Qlm, double cost, double phi);
+
+/* Status of the last call made from this thread: 0 = success.  Additive: SHTns has no
+   status getter (its GPU calls print and return void, sht_gpu.cu:566, 610-615). */
+int shtns_last_error(void);
+const char *shtns_last_error_string(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,64 @@
+/*
+ * shtns_hip.h -- device-pointer transforms of shtns_amd (stands in for SHTns 3.5.2's
+ * shtns_cuda.h; hipStream_t replaces cudaStream_t, both opaque stream handles).
+ *
+ * All transforms take device pointers, enqueue their kernels on the configuration's
+ * compute stream and return without synchronizing.  As in SHTns, one configuration must
+ * not be used from two host threads at the same time (shtns_cuda.h:28-31); clone it.
+ * Layouts are SHTns's: spectral cplx[nlm] (LM order), spatial double[nphi][nlat]
+ * (theta-contiguous, north first).  Batched entry points process nfields independent
+ * fields whose arrays sit dist elements apart.
+ */
+#ifndef SHTNS_AMD_SHTNS_HIP_H
+#define SHTNS_AMD_SHTNS_HIP_H
+
+#include <hip/hip_runtime_api.h>
+
+#include "shtns.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- transforms on device memory (shtns_cuda.h:38-52) ---- */
+void cu_spat_to_SH(shtns_cfg shtns, double *Vr, cplx *Qlm, int ltr);                       /* :38 */
+void cu_SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr, int ltr);                       /* :42 */
+void cu_spat_to_SHsphtor(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr); /* :46 */
+void cu_SHsphtor_to_spat(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp, int ltr); /* :48 */
+
+/* ---- initialization (shtns_cuda.h:63-72, sht_gpu.cu:227-336) ---- */
+shtns_cfg cushtns_clone(shtns_cfg shtns, hipStream_t compute_stream, hipStream_t transfer_stream); /* :63 */
+int cushtns_init_gpu(shtns_cfg shtns);                                                     /* :69 */
+void cushtns_set_streams(shtns_cfg shtns, hipStream_t compute_stream, hipStream_t transfer_stream); /* :72 */
+void cushtns_release_gpu(shtns_cfg shtns);                                                 /* sht_gpu.cu:137 */
+
+/* ---- batched transforms (additive: SHTns batches only inside its kernels, NFIELDS<=2,
+ *      sht_gpu.cu:477-493).  Field f reads/writes Qlm + f*spec_dist and Vr + f*spat_dist
+ *      (in elements of cplx / double).  spec_dist >= nlm, spat_dist >= nspat. ---- */
+void cu_SH_to_spat_batch(shtns_cfg, cplx *Qlm, double *Vr, int ltr, int nfields, long spec_dist, long spat_dist);
+void cu_spat_to_SH_batch(shtns_cfg, double *Vr, cplx *Qlm, int ltr, int nfields, long spat_dist, long spec_dist);
+void cu_SHsphtor_to_spat_batch(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp, int ltr,
+                               int nfields, long spec_dist, long spat_dist);
+void cu_spat_to_SHsphtor_batch(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr,
+                               int nfields, long spat_dist, long spec_dist);
+
+/* Adjoint (transpose) operators, for reverse-mode differentiation (additive).
+ *   cu_SH_to_spat_adjoint:  Qlm = S^T(Vr)   with S the synthesis operator of SH_to_spat
+ *   cu_spat_to_SH_adjoint:  Vr  = A^T(Qlm)  with A the analysis operator of spat_to_SH
+ * for the real inner products <Q,Q'> = sum Re(conj(Q) Q') over stored (l,m) and
+ * <V,V'> = sum V V' over grid points.  Batched like the functions above. */
+void cu_SH_to_spat_adjoint_batch(shtns_cfg, double *Vr, cplx *Qlm, int ltr, int nfields, long spat_dist, long spec_dist);
+void cu_spat_to_SH_adjoint_batch(shtns_cfg, cplx *Qlm, double *Vr, int ltr, int nfields, long spec_dist, long spat_dist);
+
+/* Per-kernel timing with HIP events on the configuration's compute stream (additive, no
+ * SHTns counterpart; used by the benchmark).  shtns_profile(cfg, 1) clears and enables;
+ * shtns_profile_get fills ms[k] (total milliseconds) and count[k] (launches) for
+ * k = 0 Legendre synthesis, 1 Legendre analysis, 2 FFT synthesis, 3 FFT analysis,
+ * 4 vector conversions, waiting for the recorded events; returns the number of kinds (5). */
+void shtns_profile(shtns_cfg, int enable);
+int shtns_profile_get(shtns_cfg, double *ms, int *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,145 @@
+"""ctypes binding of the CPU oracle (oracle/sht_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and the
+``cpu_baseline`` leg of ``bench.py`` as the checker.  The product package
+``shtns_amd`` never imports it.
+
+Arrays follow SHTns's layouts (shtns.h:104-106, doc/main.dox:174-189):
+spectral ``complex128[nlm]`` in LM order, spatial ``float64[nphi, nlat]``
+(theta-contiguous, north first).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so with the committed Makefile."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE] + (["-B"] if force else []))
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        lib = ctypes.CDLL(_LIB_PATH)
+        P = ctypes.c_void_p
+        I = ctypes.c_int
+        D = ctypes.c_double
+        lib.orc_create.restype = P
+        lib.orc_create.argtypes = [I, I, I, I, I]
+        lib.orc_destroy.argtypes = [P]
+        lib.orc_nlm.restype = ctypes.c_long
+        lib.orc_nlm.argtypes = [P]
+        lib.orc_set_threads.argtypes = [I]
+        lib.orc_grid.argtypes = [P, P, P, P]
+        lib.orc_legendre.argtypes = [P, I, D, D, I, P]
+        lib.orc_SH_to_spat.argtypes = [P, P, P, I]
+        lib.orc_spat_to_SH.argtypes = [P, P, P, I]
+        lib.orc_SHsphtor_to_spat.argtypes = [P, P, P, P, P, I]
+        lib.orc_spat_to_SHsphtor.argtypes = [P, P, P, P, P, I]
+        lib.orc_fft.argtypes = [P, I, I]
+        _lib = lib
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_threads(n):
+    _load().orc_set_threads(int(n))
+
+
+def fft(x, sign=-1):
+    """Unnormalized complex DFT of a 1-D complex array (sign -1 forward)."""
+    a = np.ascontiguousarray(x, dtype=np.complex128).copy()
+    _load().orc_fft(_p(a), a.size, int(sign))
+    return a
+
+
+class Oracle:
+    """One Gauss-grid configuration of the oracle."""
+
+    def __init__(self, lmax, mmax=None, mres=1, nlat=None, nphi=None):
+        if mmax is None:
+            mmax = lmax // mres
+        if nlat is None:
+            nlat = lmax + 1 + ((lmax + 1) & 1)
+        if nphi is None:
+            nphi = 2 * mmax + 2
+        self.lmax, self.mmax, self.mres, self.nlat, self.nphi = lmax, mmax, mres, nlat, nphi
+        lib = _load()
+        self._h = lib.orc_create(lmax, mmax, mres, nlat, nphi)
+        if not self._h:
+            raise ValueError("invalid sizes for oracle config")
+        self.nlm = int(lib.orc_nlm(self._h))
+        ct = np.empty(nlat)
+        st = np.empty(nlat)
+        w = np.empty(nlat)
+        lib.orc_grid(self._h, _p(ct), _p(st), _p(w))
+        self.cos_theta, self.sin_theta, self.gauss_w = ct, st, w
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _load().orc_destroy(self._h)
+            self._h = None
+
+    def idx(self, l, m):
+        im = m // self.mres
+        return (im * (2 * self.lmax + 2 - (im + 1) * self.mres)) // 2 + l
+
+    def legendre(self, im, x, sint=None, ltop=None):
+        if sint is None:
+            sint = np.sqrt((1.0 - x) * (1.0 + x))
+        m = im * self.mres
+        if ltop is None:
+            ltop = self.lmax
+        y = np.empty(ltop - m + 1)
+        _load().orc_legendre(self._h, im, float(x), float(sint), ltop, _p(y))
+        return y
+
+    def _spec(self, q):
+        q = np.ascontiguousarray(q, dtype=np.complex128)
+        assert q.shape == (self.nlm,)
+        return q
+
+    def _spat(self, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        assert v.shape == (self.nphi, self.nlat)
+        return v
+
+    def synth(self, qlm, ltr=-1):
+        q = self._spec(qlm)
+        v = np.empty((self.nphi, self.nlat))
+        _load().orc_SH_to_spat(self._h, _p(q), _p(v), int(ltr))
+        return v
+
+    def analys(self, vr, ltr=-1):
+        v = self._spat(vr)
+        q = np.empty(self.nlm, dtype=np.complex128)
+        _load().orc_spat_to_SH(self._h, _p(v), _p(q), int(ltr))
+        return q
+
+    def synth_sphtor(self, slm, tlm, ltr=-1):
+        s, t = self._spec(slm), self._spec(tlm)
+        vt = np.empty((self.nphi, self.nlat))
+        vp = np.empty((self.nphi, self.nlat))
+        _load().orc_SHsphtor_to_spat(self._h, _p(s), _p(t), _p(vt), _p(vp), int(ltr))
+        return vt, vp
+
+    def analys_sphtor(self, vt, vp, ltr=-1):
+        a, b = self._spat(vt), self._spat(vp)
+        s = np.empty(self.nlm, dtype=np.complex128)
+        t = np.empty(self.nlm, dtype=np.complex128)
+        _load().orc_spat_to_SHsphtor(self._h, _p(a), _p(b), _p(s), _p(t), int(ltr))
+        return s, t

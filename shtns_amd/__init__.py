@@ -1,0 +1,336 @@
+"""shtns_amd -- spherical harmonic transforms on one AMD MI355X (gfx950).
+
+Python mirror of SHTns's ``shtns.sht`` class (shtns_numpy.i:160-420), backed by the C-ABI
+library ``libshtns_amd.so`` (include/shtns.h, include/shtns_hip.h):
+
+* numpy arrays go through the host-pointer API (copied to the GPU and back);
+* torch CUDA tensors go through the device-pointer API on torch's current stream, with any
+  number of leading batch dimensions (one launch for the whole batch).
+
+Spectral arrays are complex128 ``[..., nlm]`` in SHTns's LM order (``sht.idx(l, m)``);
+spatial arrays are float64 ``[..., *spat_shape]`` with ``spat_shape = (nphi, nlat)`` for
+theta-contiguous grids and ``(nlat, nphi)`` for phi-contiguous grids (the Python default,
+as in SHTns, shtns_numpy.i:214).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+# constants (shtns.h:48-75)
+sht_orthonormal, sht_fourpi, sht_schmidt, sht_for_rotations = 0, 1, 2, 3
+SHT_NO_CS_PHASE = 256 * 4
+SHT_REAL_NORM = 256 * 8
+sht_gauss, sht_auto, sht_reg_fast, sht_reg_dct, sht_quick_init, sht_reg_poles, sht_gauss_fly = range(7)
+SHT_NATIVE_LAYOUT = 0
+SHT_THETA_CONTIGUOUS = 256
+SHT_PHI_CONTIGUOUS = 256 * 2
+SHT_SOUTH_POLE_FIRST = 256 * 32
+SHT_SCALAR_ONLY = 256 * 16
+SHT_LOAD_SAVE_CFG = 256 * 64
+SHT_ALLOW_GPU = 256 * 128
+SHT_ALLOW_PADDING = 256 * 256
+
+__all__ = ["sht", "nlm_calc", "build_info"]
+
+
+def nlm_calc(lmax, mmax, mres):
+    return int(_lib.load().nlm_calc(lmax, mmax, mres))
+
+
+def build_info():
+    return _lib.load().shtns_get_build_info().decode()
+
+
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(a):
+    if _is_torch(a):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class sht:
+    """Spherical harmonic transform of given sizes (SHTns ``shtns.sht``)."""
+
+    def __init__(self, lmax, mmax=-1, mres=1, norm=sht_orthonormal, nthreads=0):
+        if lmax < 2:
+            raise ValueError("lmax < 2 not allowed")
+        if mres <= 0:
+            raise ValueError("mres <= 0 invalid")
+        if mmax < 0:
+            mmax = lmax // mres
+        if mmax * mres > lmax:
+            raise ValueError("lmax < mmax*mres invalid")
+        lib = _lib.load()
+        lib.shtns_use_threads(nthreads)
+        self._cfg = lib.shtns_create(lmax, mmax, mres, norm)
+        if not self._cfg:
+            _lib.check_error("shtns_create")
+            raise RuntimeError("shtns_create failed")
+        info = self._cfg.contents
+        self.lmax, self.mmax, self.mres, self.nlm = info.lmax, info.mmax, info.mres, info.nlm
+        self.nlat = self.nphi = self.nspat = 0
+        self.l = np.zeros(self.nlm, dtype=np.int32)
+        self.m = np.zeros(self.nlm, dtype=np.int32)
+        for mm in range(0, self.mmax * self.mres + 1, self.mres):
+            for ll in range(mm, self.lmax + 1):
+                i = self.idx(ll, mm)
+                self.l[i], self.m[i] = ll, mm
+        self.l.flags.writeable = False
+        self.m.flags.writeable = False
+        self._layout = 0
+
+    def __del__(self):
+        cfg = getattr(self, "_cfg", None)
+        if cfg:
+            try:
+                _lib.load().shtns_destroy(cfg)
+            except Exception:
+                pass
+            self._cfg = None
+
+    # ---- grid ----
+    def set_grid(self, nlat=0, nphi=0, flags=sht_quick_init, polar_opt=1.0e-10, nl_order=1):
+        """Choose the Gauss grid (shtns_numpy.i:205-222).  Defaults to phi-contiguous
+        layout unless SHT_THETA_CONTIGUOUS is given, as SHTns's Python binding does."""
+        if nlat != 0 and nlat <= self.lmax:
+            raise ValueError("nlat <= lmax")
+        if nphi != 0 and nphi <= self.mmax * 2:
+            raise ValueError("nphi <= 2*mmax")
+        if not (flags & SHT_THETA_CONTIGUOUS):
+            flags |= SHT_PHI_CONTIGUOUS
+        lib = _lib.load()
+        a, b = ctypes.c_int(nlat), ctypes.c_int(nphi)
+        n = lib.shtns_set_grid_auto(self._cfg, flags, polar_opt, nl_order, ctypes.byref(a), ctypes.byref(b))
+        if n == 0:
+            _lib.check_error("set_grid")
+        info = self._cfg.contents
+        self.nlat, self.nphi, self.nspat = info.nlat, info.nphi, info.nspat
+        self._layout = flags & (SHT_PHI_CONTIGUOUS | SHT_THETA_CONTIGUOUS)
+        self.cos_theta = np.ctypeslib.as_array(info.ct, shape=(self.nlat,)).copy()
+        self.cos_theta.flags.writeable = False
+        if self._layout & SHT_PHI_CONTIGUOUS:
+            self.spat_shape = (self.nlat, self.nphi)
+        else:
+            self.spat_shape = (self.nphi, self.nlat)
+        return self.nlat, self.nphi
+
+    def gauss_wts(self):
+        w = np.empty(self.nlat // 2)
+        _lib.load().shtns_gauss_wts(self._cfg, _ptr(w))
+        return w
+
+    def print_info(self):
+        _lib.load().shtns_print_cfg(self._cfg)
+
+    def sh00_1(self):
+        return _lib.load().sh00_1(self._cfg)
+
+    def sh10_ct(self):
+        return _lib.load().sh10_ct(self._cfg)
+
+    def sh11_st(self):
+        return _lib.load().sh11_st(self._cfg)
+
+    def shlm_e1(self, l, m):
+        return _lib.load().shlm_e1(self._cfg, l, m)
+
+    def idx(self, l, m):
+        if l > self.lmax:
+            raise ValueError("l invalid")
+        if m > l or m > self.mmax * self.mres or m % self.mres != 0:
+            raise ValueError("m invalid")
+        im = m // self.mres
+        return (im * (2 * self.lmax + 2 - (im + 1) * self.mres)) // 2 + l
+
+    def spec_array(self, im=-1):
+        if im < 0:
+            return np.zeros(self.nlm, dtype=complex)
+        return np.zeros(self.lmax + 1 - im * self.mres, dtype=complex)
+
+    def spat_array(self):
+        if self.nlat == 0:
+            raise RuntimeError("Grid not set. Call .set_grid() method.")
+        return np.zeros(self.spat_shape)
+
+    def legendre(self, im, x, lmax=None):
+        """Normalized associated Legendre functions Y_l^m(x), l = m..lmax."""
+        if lmax is None:
+            lmax = self.lmax
+        y = np.empty(lmax - im * self.mres + 1)
+        _lib.load().legendre_sphPlm_array(self._cfg, lmax, im, float(x), _ptr(y))
+        return y
+
+    def SH_to_point(self, qlm, cost, phi):
+        q = np.ascontiguousarray(qlm, dtype=np.complex128)
+        return _lib.load().SH_to_point(self._cfg, _ptr(q), float(cost), float(phi))
+
+    # ---- checks ----
+    def _need_grid(self):
+        if self.nlat == 0:
+            raise RuntimeError("Grid not set. Call .set_grid() method.")
+
+    def _check_spec(self, q):
+        if q.shape[-1] != self.nlm:
+            raise RuntimeError("spectral array has wrong size.")
+
+    def _check_spat(self, v):
+        if tuple(v.shape[-2:]) != tuple(self.spat_shape):
+            raise RuntimeError("spatial array has wrong shape.")
+
+    # ---- numpy (host) transforms, SHTns names ----
+    def SH_to_spat(self, Qlm, Vr, ltr=-1):
+        self._need_grid()
+        lib = _lib.load()
+        lib.SH_to_spat_l(self._cfg, _ptr(Qlm), _ptr(Vr), self.lmax if ltr < 0 else ltr)
+        _lib.check_error("SH_to_spat")
+
+    def spat_to_SH(self, Vr, Qlm, ltr=-1):
+        self._need_grid()
+        lib = _lib.load()
+        lib.spat_to_SH_l(self._cfg, _ptr(Vr), _ptr(Qlm), self.lmax if ltr < 0 else ltr)
+        _lib.check_error("spat_to_SH")
+
+    def SHsphtor_to_spat(self, Slm, Tlm, Vt, Vp, ltr=-1):
+        self._need_grid()
+        _lib.load().SHsphtor_to_spat_l(self._cfg, _ptr(Slm), _ptr(Tlm), _ptr(Vt), _ptr(Vp), self.lmax if ltr < 0 else ltr)
+        _lib.check_error("SHsphtor_to_spat")
+
+    def spat_to_SHsphtor(self, Vt, Vp, Slm, Tlm, ltr=-1):
+        self._need_grid()
+        _lib.load().spat_to_SHsphtor_l(self._cfg, _ptr(Vt), _ptr(Vp), _ptr(Slm), _ptr(Tlm), self.lmax if ltr < 0 else ltr)
+        _lib.check_error("spat_to_SHsphtor")
+
+    # ---- generic entry points (numpy or torch) ----
+    def synth(self, *arg, ltr=-1):
+        """v = synth(qlm) ; vt, vp = synth(slm, tlm)   (shtns_numpy.i:396-420)."""
+        self._need_grid()
+        n = len(arg)
+        if n not in (1, 2):
+            raise RuntimeError("1 or 2 spectral arrays required.")
+        if _is_torch(arg[0]):
+            return self._synth_torch(arg, ltr)
+        q = []
+        for a in arg:
+            self._check_spec(a)
+            if a.dtype != np.complex128:
+                raise RuntimeError("spectral array should be dtype=complex.")
+            q.append(np.ascontiguousarray(a))
+        if n == 1:
+            vr = np.empty(self.spat_shape)
+            self.SH_to_spat(q[0], vr, ltr)
+            return vr
+        vt, vp = np.empty(self.spat_shape), np.empty(self.spat_shape)
+        self.SHsphtor_to_spat(q[0], q[1], vt, vp, ltr)
+        return vt, vp
+
+    def analys(self, *arg, ltr=-1):
+        """qlm = analys(v) ; slm, tlm = analys(vt, vp)."""
+        self._need_grid()
+        n = len(arg)
+        if n not in (1, 2):
+            raise RuntimeError("1 or 2 spatial arrays required.")
+        if _is_torch(arg[0]):
+            return self._analys_torch(arg, ltr)
+        v = []
+        for a in arg:
+            self._check_spat(a)
+            v.append(np.ascontiguousarray(a, dtype=np.float64))
+        if n == 1:
+            q = self.spec_array()
+            self.spat_to_SH(v[0], q, ltr)
+            return q
+        s, t = self.spec_array(), self.spec_array()
+        self.spat_to_SHsphtor(v[0], v[1], s, t, ltr)
+        return s, t
+
+    # ---- torch (device) transforms ----
+    def _stream(self, t):
+        import torch
+        s = torch.cuda.current_stream(t.device).cuda_stream
+        _lib.load().cushtns_set_streams(self._cfg, ctypes.c_void_p(s), ctypes.c_void_p(s))
+
+    def _torch_spec(self, q):
+        import torch
+        if not q.is_cuda:
+            raise RuntimeError("torch tensors must be on the GPU")
+        if q.dtype != torch.complex128:
+            raise RuntimeError("spectral tensor should be torch.complex128")
+        self._check_spec(q)
+        return q.contiguous()
+
+    def _torch_spat(self, v):
+        import torch
+        if not v.is_cuda:
+            raise RuntimeError("torch tensors must be on the GPU")
+        if v.dtype != torch.float64:
+            raise RuntimeError("spatial tensor should be torch.float64")
+        self._check_spat(v)
+        return v.contiguous()
+
+    def _synth_torch(self, arg, ltr):
+        import torch
+        q = [self._torch_spec(a) for a in arg]
+        batch = tuple(q[0].shape[:-1])
+        nf = int(np.prod(batch)) if batch else 1
+        self._stream(q[0])
+        lib = _lib.load()
+        lt = self.lmax if ltr < 0 else ltr
+        out = [torch.empty(batch + tuple(self.spat_shape), dtype=torch.float64, device=q[0].device) for _ in q]
+        if nf > 0:
+            if len(q) == 1:
+                lib.cu_SH_to_spat_batch(self._cfg, _ptr(q[0]), _ptr(out[0]), lt, nf, self.nlm, self.nspat)
+            else:
+                lib.cu_SHsphtor_to_spat_batch(self._cfg, _ptr(q[0]), _ptr(q[1]), _ptr(out[0]), _ptr(out[1]), lt, nf,
+                                              self.nlm, self.nspat)
+            _lib.check_error("synth")
+        return out[0] if len(q) == 1 else tuple(out)
+
+    def _analys_torch(self, arg, ltr):
+        import torch
+        v = [self._torch_spat(a) for a in arg]
+        batch = tuple(v[0].shape[:-2])
+        nf = int(np.prod(batch)) if batch else 1
+        self._stream(v[0])
+        lib = _lib.load()
+        lt = self.lmax if ltr < 0 else ltr
+        out = [torch.empty(batch + (self.nlm,), dtype=torch.complex128, device=v[0].device) for _ in v]
+        if nf > 0:
+            if len(v) == 1:
+                lib.cu_spat_to_SH_batch(self._cfg, _ptr(v[0]), _ptr(out[0]), lt, nf, self.nspat, self.nlm)
+            else:
+                lib.cu_spat_to_SHsphtor_batch(self._cfg, _ptr(v[0]), _ptr(v[1]), _ptr(out[0]), _ptr(out[1]), lt, nf,
+                                              self.nspat, self.nlm)
+            _lib.check_error("analys")
+        return out[0] if len(v) == 1 else tuple(out)
+
+    # adjoint operators on torch tensors (used by shtns_amd.autograd)
+    def synth_adjoint(self, v, ltr=-1):
+        import torch
+        v = self._torch_spat(v)
+        batch = tuple(v.shape[:-2])
+        nf = int(np.prod(batch)) if batch else 1
+        self._stream(v)
+        out = torch.empty(batch + (self.nlm,), dtype=torch.complex128, device=v.device)
+        if nf > 0:
+            _lib.load().cu_SH_to_spat_adjoint_batch(self._cfg, _ptr(v), _ptr(out), self.lmax if ltr < 0 else ltr, nf,
+                                                    self.nspat, self.nlm)
+            _lib.check_error("synth_adjoint")
+        return out
+
+    def analys_adjoint(self, q, ltr=-1):
+        import torch
+        q = self._torch_spec(q)
+        batch = tuple(q.shape[:-1])
+        nf = int(np.prod(batch)) if batch else 1
+        self._stream(q)
+        out = torch.empty(batch + tuple(self.spat_shape), dtype=torch.float64, device=q.device)
+        if nf > 0:
+            _lib.load().cu_spat_to_SH_adjoint_batch(self._cfg, _ptr(q), _ptr(out), self.lmax if ltr < 0 else ltr, nf,
+                                                    self.nlm, self.nspat)
+            _lib.check_error("analys_adjoint")
+        return out

@@ -1,0 +1,984 @@
+// sht_host.cpp -- host side of shtns_amd: the SHTns C API (shtns.h / shtns_hip.h) on top of
+// the gfx950 kernels in sht_kernels.hip.
+//
+// Host math is FP64 (long double where it buys accuracy for the tables):
+//   Gauss nodes/weights          restates sht_legendre.c:763-811 and sht_init.c:560-649
+//   a_mm seeds                   restates sht_legendre.c:584-613
+//   Ishioka recurrence tables    same functions as sht_legendre.c:673-734, derived here from
+//                                x P_l = e_{l+1} P_{l+1} + e_l P_{l-1} (see build_order_tables)
+// Every transform runs on the GPU; without a usable GPU the transform entry points report
+// an error (shtns_last_error) and leave their outputs untouched -- there is no CPU path.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "sht_internal.h"
+
+using namespace shtns_amd;
+
+// ------------------------------------------------------------------------------------------
+// errors and global state
+// ------------------------------------------------------------------------------------------
+static std::mutex g_mutex;
+static shtns_info *g_cfg_list = nullptr;   // every live config (for shtns_reset)
+static int g_verbose = 0;
+static int g_gpu_request = 0;              // device chosen by shtns_use_gpu (-1: none)
+static thread_local int g_last_err = 0;
+static thread_local char g_last_msg[256] = "";
+
+static void set_error(int code, const char *fmt, ...) {
+    g_last_err = code;
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_last_msg, sizeof(g_last_msg), fmt, ap);
+    va_end(ap);
+    if (code) fprintf(stderr, "*** [shtns_amd] %s\n", g_last_msg);
+}
+static void clear_error() {
+    g_last_err = 0;
+    g_last_msg[0] = 0;
+}
+
+// invalid sizes abort the program, as SHTns's shtns_runerr does (sht_init.c:68-72)
+[[noreturn]] static void runerr(const char *msg) {
+    printf("*** [shtns_amd] Run-time error : %s\n", msg);
+    fflush(stdout);
+    exit(1);
+}
+
+#define HIP_OK(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            set_error(2, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__); \
+            return;                                                                      \
+        }                                                                                \
+    } while (0)
+#define HIP_OK_RET(call, ret)                                                            \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            set_error(2, "HIP error %s at %s:%d", hipGetErrorString(e_), __FILE__, __LINE__); \
+            return ret;                                                                  \
+        }                                                                                \
+    } while (0)
+
+static void prof_clear(shtns_cfg c);
+
+extern "C" int shtns_last_error(void) { return g_last_err; }
+extern "C" const char *shtns_last_error_string(void) { return g_last_msg; }
+
+// ------------------------------------------------------------------------------------------
+// sizes
+// ------------------------------------------------------------------------------------------
+extern "C" long nlm_calc(long lmax, long mmax, long mres) {
+    if (mmax * mres > lmax) mmax = lmax / mres;
+    return (mmax + 1) * (lmax + 1) - ((mmax * mres) * (mmax + 1)) / 2;
+}
+
+extern "C" long nlm_cplx_calc(long lmax, long mmax, long mres) {
+    if (mmax * mres > lmax) mmax = lmax / mres;
+    return (2 * mmax + 1) * (lmax + 1) - (mmax * mres) * (mmax + 1);
+}
+
+extern "C" void shtns_verbose(int v) { g_verbose = v; }
+
+extern "C" const char *shtns_get_build_info(void) {
+    return "[shtns_amd 0.1] SHTns 3.5.2 API, gfx950 HIP kernels (FP64 MFMA Legendre, LDS mixed-radix FFT), built " __DATE__;
+}
+extern "C" void shtns_print_version(void) { printf("%s\n", shtns_get_build_info()); }
+
+extern "C" void shtns_print_cfg(shtns_cfg c) {
+    printf("Lmax=%d, Mmax*Mres=%d, Mres=%d, Nlm=%d  [GPU %d, orthonormalized]\n", c->lmax, c->mmax * c->mres, c->mres,
+           c->nlm, c->device);
+    if (c->ct == nullptr) return;
+    printf("Gauss grid : Nlat=%d, Nphi=%d  (%s)\n", c->nlat, c->nphi,
+           (c->layout & SHT_PHI_CONTIGUOUS) ? "phi-contiguous" : "theta-contiguous");
+}
+
+// ------------------------------------------------------------------------------------------
+// host math
+// ------------------------------------------------------------------------------------------
+// Gauss-Legendre nodes x (descending, north first) and weights (sum 2), Newton iteration
+// from the same initial guess as SHTns (sht_legendre.c:763-811).
+static void gauss_nodes(double *x, double *st, double *w, int n) {
+    const double eps = 2.3e-16;
+    for (long i = 0; i < n / 2; ++i) {
+        double z = (1.0 - (n - 1.) / (8. * n * n * n)) * cos((M_PI * (4 * i + 3)) / (4 * n + 2));
+        double z1, pp, p1, p2;
+        int k = 10;
+        do {
+            p1 = z;
+            p2 = 1.0;
+            for (long l = 2; l <= n; ++l) {
+                const double p3 = p2;
+                p2 = p1;
+                p1 = ((2 * l - 1) * z * p2 - (l - 1) * p3) / l;
+            }
+            pp = n * (p2 - z * p1);
+            z1 = z;
+            z -= p1 * (1. - z * z) / pp;
+        } while ((fabs(z - z1) > (z1 + z) * 0.5 * eps) && (--k > 0));
+        const double s2 = 1. - z * z;
+        x[i] = z;
+        x[n - 1 - i] = -z;
+        w[i] = 2.0 * s2 / (pp * pp);
+        w[n - 1 - i] = w[i];
+        st[i] = sqrt(s2);
+        st[n - 1 - i] = st[i];
+    }
+}
+
+// Y_m^m prefactor sqrt(prod_{k<=m} (2k+1)/(2k) / (4 pi)) with Condon-Shortley sign (-1)^m.
+static long double seed_amm(int m) {
+    long double t = 0.25L / 3.14159265358979323846264338327950288L;
+    for (int k = 1; k <= m; ++k) t *= ((long double)k + 0.5L) / k;
+    long double s = sqrtl(t);
+    return (m & 1) ? -s : s;
+}
+
+static long double eps_l(int l, int m) {
+    if (l <= m) return 0.0L;
+    return sqrtl((long double)((long)(l - m) * (l + m)) / (long double)((long)(2 * l + 1) * (2 * l - 1)));
+}
+
+// Recurrence tables of one order m, steps i = 0..nst-1 (degrees m .. m+2*nst-1).
+// With e_l as above, u_i = P_{m+2i}, v_i = P_{m+2i+1}/x:
+//     u_i = e_{m+2i+1} v_i + e_{m+2i} v_{i-1},      x^2 v_i = e_{m+2i+2} u_{i+1} + e_{m+2i+1} u_i.
+// Writing v_i = d_i z_i with d_i d_{i+1} = (-1)^i / (e_{m+2i+2} e_{m+2i+3}) turns this into
+//     z_{i+1} = (alpha_i x^2 + beta_i) z_i + z_{i-1},
+//     alpha_i = (-1)^i d_i^2,   beta_i = -alpha_i (e_{m+2i+1}^2 + e_{m+2i+2}^2),
+// with z_0 = sin^m(theta), z_{-1} = 0, d_0 = a_mm / e_{m+1}; then
+//     P_{m+2i} = ue_i z_i + ue1_{i-1} z_{i-1},  ue_i = e_{m+2i+1} d_i,  ue1_i = e_{m+2i+2} d_i,
+//     P_{m+2i+1} = x uo_i z_i,                  uo_i = d_i.
+static void build_order_tables(int m, int nst, double2 *ab, double4 *pm) {
+    long double d = seed_amm(m) / eps_l(m + 1, m);
+    for (int i = 0; i < nst; ++i) {
+        const long double e1 = eps_l(m + 2 * i + 1, m), e2 = eps_l(m + 2 * i + 2, m), e3 = eps_l(m + 2 * i + 3, m);
+        const long double sgn = (i & 1) ? -1.0L : 1.0L;
+        const long double alpha = sgn * d * d;
+        const long double beta = -alpha * (e1 * e1 + e2 * e2);
+        ab[i] = make_double2((double)alpha, (double)beta);
+        pm[i] = make_double4((double)(e1 * d), (double)(e2 * d), (double)d, 0.0);
+        d = sgn / (e2 * e3 * d);
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// config lifetime
+// ------------------------------------------------------------------------------------------
+extern "C" shtns_cfg shtns_create(int lmax, int mmax, int mres, enum shtns_norm norm) {
+    clear_error();
+    if (lmax < 2) runerr("lmax must be at least 2");
+    if (lmax > 65534) runerr("lmax too large");
+    if (mres <= 0) runerr("MRES must be > 0");
+    if (mmax < 0 || mmax * mres > lmax) runerr("MMAX*MRES should not exceed LMAX");
+    if ((int)norm != (int)sht_orthonormal) {
+        set_error(1, "only the orthonormal normalization with Condon-Shortley phase is implemented (norm=%d)", (int)norm);
+        return nullptr;
+    }
+    shtns_info *c = new shtns_info();
+    c->lmax = (unsigned short)lmax;
+    c->mmax = (unsigned short)mmax;
+    c->mres = (unsigned short)mres;
+    c->nlm = (unsigned int)nlm_calc(lmax, mmax, mres);
+    c->nlm_cplx = (unsigned int)nlm_cplx_calc(lmax, mmax, mres);
+    c->norm = (int)norm;
+    c->li = new unsigned short[2 * (size_t)c->nlm];
+    c->mi = c->li + c->nlm;
+    long lm = 0;
+    for (int im = 0; im <= mmax; ++im)
+        for (int l = im * mres; l <= lmax; ++l) {
+            c->li[lm] = (unsigned short)l;
+            c->mi[lm] = (unsigned short)(im * mres);
+            ++lm;
+        }
+    std::lock_guard<std::mutex> g(g_mutex);
+    c->next = g_cfg_list;
+    g_cfg_list = c;
+    return c;
+}
+
+static void free_device(shtns_info *c) {
+    if (!c->gpu_ready) return;
+    (void)hipSetDevice(c->device);
+    if (c->clone_of == nullptr) {
+        DeviceTables &d = c->d;
+        void *ptrs[] = {d.x, d.x2, d.st, d.wq, d.step_ofs, d.ab, d.pm, d.i0, d.z0, d.roots, d.fft_pos};
+        for (void *p : ptrs)
+            if (p) (void)hipFree(p);
+    }
+    if (c->scratch) (void)hipFree(c->scratch);
+    if (c->own_streams) {
+        if (c->comp_stream) (void)hipStreamDestroy(c->comp_stream);
+    }
+    c->d = DeviceTables();
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+    c->gpu_ready = false;
+}
+
+extern "C" void cushtns_release_gpu(shtns_cfg c) { free_device(c); }
+
+extern "C" void shtns_unset_grid(shtns_cfg c) {
+    prof_clear(c);
+    free_device(c);
+    delete[] c->ct;
+    c->ct = nullptr;
+    c->st = nullptr;
+    c->wg.clear();
+    c->nlat = 0;
+    c->nlat_2 = 0;
+    c->nphi = 0;
+    c->nspat = 0;
+    c->nlat_padded = 0;
+}
+
+extern "C" void shtns_destroy(shtns_cfg c) {
+    if (!c) return;
+    {
+        std::lock_guard<std::mutex> g(g_mutex);
+        for (shtns_info *p = g_cfg_list; p; p = p->next)
+            if (p->clone_of == c) runerr("shtns_destroy: config still has live clones");
+        shtns_info **pp = &g_cfg_list;
+        while (*pp && *pp != c) pp = &(*pp)->next;
+        if (*pp) *pp = c->next;
+    }
+    shtns_unset_grid(c);
+    if (c->clone_of == nullptr) delete[] c->li;
+    delete c;
+}
+
+extern "C" void shtns_reset(void) {
+    while (true) {
+        shtns_info *c;
+        {
+            std::lock_guard<std::mutex> g(g_mutex);
+            c = g_cfg_list;
+            // destroy clones first
+            for (shtns_info *p = g_cfg_list; p; p = p->next)
+                if (p->clone_of) { c = p; break; }
+        }
+        if (!c) break;
+        shtns_destroy(c);
+    }
+}
+
+extern "C" int shtns_use_threads(int num_threads) {
+    (void)num_threads;   // transforms run on the GPU; host work is negligible
+    return 1;
+}
+
+extern "C" int shtns_use_gpu(int device_id) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0 || device_id < 0) {
+        g_gpu_request = -1;
+        return -1;
+    }
+    g_gpu_request = device_id % n;
+    if (hipSetDevice(g_gpu_request) != hipSuccess) {
+        g_gpu_request = -1;
+        return -1;
+    }
+    return g_gpu_request;
+}
+
+extern "C" void *shtns_malloc(size_t bytes) {
+    // pinned host memory when a GPU is present (as SHTns's CUDA build, sht_gpu.cu:67-81)
+    void *p = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) == hipSuccess && n > 0) {
+        if (hipHostMalloc(&p, bytes, hipHostMallocDefault) == hipSuccess) return p;
+        p = nullptr;
+    }
+    if (posix_memalign(&p, 64, bytes) != 0) return nullptr;
+    return p;
+}
+
+extern "C" void shtns_free(void *p) {
+    if (!p) return;
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) == hipSuccess && attr.type == hipMemoryTypeHost) {
+        (void)hipHostFree(p);
+        return;
+    }
+    (void)hipGetLastError();
+    free(p);
+}
+
+// ------------------------------------------------------------------------------------------
+// grid
+// ------------------------------------------------------------------------------------------
+static int factor_fft(int M, FftPlan &plan) {
+    plan.M = M;
+    plan.nstages = 0;
+    int n = M;
+    while (n % 4 == 0 && plan.nstages < 24) { plan.radix[plan.nstages++] = 4; n /= 4; }
+    const int small[] = {2, 3, 5, 7};
+    for (int p : small)
+        while (n % p == 0 && plan.nstages < 24) { plan.radix[plan.nstages++] = p; n /= p; }
+    return (n == 1) ? 0 : -1;
+}
+
+// digit-reversal: position of bin k after the in-place DIF stages of `plan`
+static void fft_positions(const FftPlan &plan, std::vector<int> &pos) {
+    const int M = plan.M;
+    pos.assign(M, 0);
+    for (int k = 0; k < M; ++k) {
+        int Ns = M, kk = k, p0 = 0;
+        for (int s = 0; s < plan.nstages; ++s) {
+            const int p = plan.radix[s];
+            const int q = Ns / p;
+            p0 += (kk % p) * q;
+            kk /= p;
+            Ns = q;
+        }
+        pos[k] = p0;
+    }
+}
+
+static int device_init(shtns_info *c);
+
+extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double eps, int nl_order, int *nlat, int *nphi) {
+    clear_error();
+    const int grid = (int)flags & 255;
+    const int layout = (int)flags & ~255;
+    if (grid != sht_gauss && grid != sht_auto && grid != sht_quick_init && grid != sht_gauss_fly) {
+        set_error(1, "only Gauss grids are implemented (grid type %d)", grid);
+        return 0;
+    }
+    if (nl_order <= 0) nl_order = 1;
+    if (*nlat <= 0) {   // choose sizes like SHTns: nlat > (nl_order+1)*lmax/2, nphi > (nl_order+1)*mmax
+        int n = ((nl_order + 1) * c->lmax) / 2 + 2;
+        n += (n & 1);
+        *nlat = n;
+    }
+    if (*nphi <= 0) {
+        int n = (nl_order + 1) * c->mmax * c->mres + 2;
+        while (true) {   // smallest 2^a 3^b 5^c >= n, even
+            int t = n;
+            for (int p : {2, 3, 5})
+                while (t % p == 0) t /= p;
+            if (t == 1 && (n & 1) == 0) break;
+            ++n;
+        }
+        *nphi = n;
+    }
+    if (*nlat <= c->lmax) runerr("Nlat must be larger than Lmax");
+    if (*nlat & 1) {
+        set_error(1, "odd nlat is not implemented (nlat=%d)", *nlat);
+        return 0;
+    }
+    if (*nphi <= 2 * c->mmax) runerr("Nphi must be larger than 2*Mmax");
+    if (*nphi & 1) {
+        set_error(1, "odd nphi is not implemented (nphi=%d)", *nphi);
+        return 0;
+    }
+    FftPlan plan{};
+    if (factor_fft(*nphi / 2, plan) != 0) {
+        set_error(1, "nphi/2 must factor into 2,3,5,7 (nphi=%d)", *nphi);
+        return 0;
+    }
+    shtns_unset_grid(c);
+    c->nlat = *nlat;
+    c->nlat_2 = (unsigned short)(*nlat / 2);
+    c->nphi = *nphi;
+    c->nspat = c->nlat * c->nphi;
+    c->nlat_padded = c->nlat;
+    c->layout = (layout & SHT_PHI_CONTIGUOUS) ? SHT_PHI_CONTIGUOUS : SHT_THETA_CONTIGUOUS;
+    if (layout & SHT_SOUTH_POLE_FIRST) c->layout |= SHT_SOUTH_POLE_FIRST;
+    c->eps = eps;
+    c->ct = new double[2 * (size_t)c->nlat];
+    c->st = c->ct + c->nlat;
+    std::vector<double> w(c->nlat);
+    gauss_nodes(c->ct, c->st, w.data(), c->nlat);
+    {   // quadrature sanity sums (sht_init.c:607-624)
+        double s = 0, x2 = 0, st2 = 0;
+        for (int i = 0; i < c->nlat_2; ++i) {
+            const int i2 = c->nlat - 1 - i;
+            s += w[i] + w[i2];
+            x2 += w[i] * c->ct[i] * c->ct[i] + w[i2] * c->ct[i2] * c->ct[i2];
+            st2 += w[i] * c->st[i] * c->st[i] + w[i2] * c->st[i2] * c->st[i2];
+        }
+        if (fabs(s - 2.0) + fabs(x2 * 1.5 - 1.) + fabs(st2 * 0.75 - 1.) > 1e-14) runerr("Bad quadrature accuracy.");
+    }
+    if (layout & SHT_SOUTH_POLE_FIRST)
+        for (unsigned i = 0; i < c->nlat; ++i) c->ct[i] = -c->ct[i];
+    c->wg.assign(w.begin(), w.begin() + c->nlat_2);
+    c->d.plan = plan;
+
+    // GPU tables (a config is usable for queries without a GPU; transforms then fail)
+    if (g_gpu_request >= 0) {
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {
+            if (device_init(c) < 0) fprintf(stderr, "*** [shtns_amd] GPU initialization failed: %s\n", g_last_msg);
+        } else {
+            (void)hipGetLastError();
+        }
+    }
+    return (int)c->nspat;
+}
+
+extern "C" int shtns_set_grid(shtns_cfg c, enum shtns_type flags, double eps, int nlat, int nphi) {
+    if (nlat == 0 || nphi == 0) runerr("nlat or nphi is zero !");
+    return shtns_set_grid_auto(c, flags, eps, 0, &nlat, &nphi);
+}
+
+extern "C" shtns_cfg shtns_init(enum shtns_type flags, int lmax, int mmax, int mres, int nlat, int nphi) {
+    shtns_cfg c = shtns_create(lmax, mmax, mres, sht_orthonormal);
+    if (c) shtns_set_grid(c, flags, 1.0e-10, nlat, nphi);
+    return c;
+}
+
+extern "C" int shtns_gauss_wts(shtns_cfg c, double *wts) {
+    if (c->wg.empty()) return 0;
+    for (int i = 0; i < c->nlat_2; ++i) wts[i] = c->wg[i];
+    return c->nlat_2;
+}
+
+extern "C" double sh00_1(shtns_cfg) { return sqrt(4. * M_PI); }
+extern "C" double sh10_ct(shtns_cfg) { return sqrt(4. * M_PI / 3.); }
+extern "C" double sh11_st(shtns_cfg) { return -sqrt(2. * M_PI / 3.); }
+extern "C" double shlm_e1(shtns_cfg, int l, int m) {
+    (void)l;
+    double x = 1.0;
+    if (m != 0) x *= sqrt(0.5);
+    return x;
+}
+
+// Normalized associated Legendre functions Y_l^m(x), l = m..lmax (GSL-compatible output,
+// shtns.h:219-227): classic three-term recurrence on the host.
+extern "C" int legendre_sphPlm_array(shtns_cfg c, const int lmax, const int im, const double x, double *yl) {
+    const int m = im * c->mres;
+    if (lmax < m || lmax > c->lmax) return -1;
+    const double s2 = (1. - x) * (1. + x);
+    long double y0 = seed_amm(m) * powl(sqrtl((long double)s2), m);
+    yl[0] = (double)y0;
+    if (lmax == m) return lmax - m + 1;
+    long double y1 = sqrtl(2.0L * m + 3) * x * y0;
+    yl[1] = (double)y1;
+    for (int l = m + 2; l <= lmax; ++l) {
+        const long double t1 = (long double)((long)(l + m) * (l - m));
+        const long double t2 = (long double)((long)(l - 1 + m) * (l - 1 - m));
+        const long double a = sqrtl(((long double)(2 * l + 1) * (2 * l - 1)) / t1);
+        const long double b = -sqrtl(((long double)(2 * l + 1) * t2) / ((long double)(2 * l - 3) * t1));
+        const long double y2 = a * x * y1 + b * y0;
+        y0 = y1;
+        y1 = y2;
+        yl[l - m] = (double)y1;
+    }
+    return lmax - m + 1;
+}
+
+// f(theta,phi) = sum_l Q_l0 Y_l0 + 2 Re sum_{m>0} Q_lm Y_lm e^{i m phi} at one point (sht_func.c:585-622).
+extern "C" double SH_to_point(shtns_cfg c, cplx *Qlm, double cost, double phi) {
+    std::vector<double> y(c->lmax + 1);
+    const double *q = reinterpret_cast<const double *>(Qlm);
+    double v = 0.0;
+    for (int im = 0; im <= c->mmax; ++im) {
+        const int m = im * c->mres;
+        legendre_sphPlm_array(c, c->lmax, im, cost, y.data());
+        const long base = LiM(c, 0, im);
+        double re = 0, ii = 0;
+        for (int l = m; l <= c->lmax; ++l) {
+            re += y[l - m] * q[2 * (base + l)];
+            ii += y[l - m] * q[2 * (base + l) + 1];
+        }
+        if (m == 0) v += re;
+        else v += 2.0 * (re * cos(m * phi) - ii * sin(m * phi));
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------------------------------
+// device initialization
+// ------------------------------------------------------------------------------------------
+template <class T>
+static int upload(T **dst, const T *src, size_t n) {
+    if (hipMalloc(dst, n * sizeof(T) + 256) != hipSuccess) return -1;
+    if (hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) return -1;
+    return 0;
+}
+
+static int device_init(shtns_info *c) {
+    int dev = (g_gpu_request >= 0) ? g_gpu_request : 0;
+    HIP_OK_RET(hipSetDevice(dev), -1);
+    c->device = dev;
+    const int nl2 = c->nlat_2, L = c->lmax, mmax = c->mmax, mres = c->mres;
+    DeviceTables &d = c->d;
+    // latitude tables
+    std::vector<double> x(nl2), x2(nl2), st(nl2), wq(nl2);
+    for (int k = 0; k < nl2; ++k) {
+        x[k] = c->ct[k];
+        x2[k] = c->ct[k] * c->ct[k];
+        st[k] = c->st[k];
+        wq[k] = c->wg[k] * 2.0 * M_PI / c->nphi;
+    }
+    // recurrence tables: slots 0..mmax (m = im*mres) and slot mmax+1 (m = 1), degrees up to lmax+1
+    std::vector<int> ofs(mmax + 3);
+    int tot = 0;
+    for (int slot = 0; slot <= mmax + 1; ++slot) {
+        const int m = (slot <= mmax) ? slot * mres : 1;
+        ofs[slot] = tot;
+        tot += (L + 1 - m) / 2 + 2;
+    }
+    ofs[mmax + 2] = tot;
+    std::vector<double2> ab(tot);
+    std::vector<double4> pm(tot);
+    for (int slot = 0; slot <= mmax + 1; ++slot) {
+        const int m = (slot <= mmax) ? slot * mres : 1;
+        build_order_tables(m, ofs[slot + 1] - ofs[slot], &ab[ofs[slot]], &pm[ofs[slot]]);
+    }
+    c->nsteps_total = tot;
+    // FFT tables
+    const int N = c->nphi;
+    std::vector<double2> roots(N);
+    for (int k = 0; k < N; ++k) {
+        const long double a = 2.0L * 3.14159265358979323846264338327950288L * k / N;
+        roots[k] = make_double2((double)cosl(a), (double)sinl(a));
+    }
+    std::vector<int> pos;
+    fft_positions(d.plan, pos);
+
+    if (upload(&d.x, x.data(), nl2) || upload(&d.x2, x2.data(), nl2) || upload(&d.st, st.data(), nl2) ||
+        upload(&d.wq, wq.data(), nl2) || upload(&d.step_ofs, ofs.data(), ofs.size()) ||
+        upload(&d.ab, ab.data(), ab.size()) || upload(&d.pm, pm.data(), pm.size()) ||
+        upload(&d.roots, roots.data(), roots.size()) || upload(&d.fft_pos, pos.data(), pos.size())) {
+        set_error(2, "device table allocation failed");
+        c->gpu_ready = true;
+        free_device(c);
+        return -1;
+    }
+    const size_t nseed = (size_t)(mmax + 2) * nl2;
+    if (hipMalloc(&d.i0, nseed * sizeof(int)) != hipSuccess || hipMalloc(&d.z0, nseed * sizeof(double2)) != hipSuccess) {
+        set_error(2, "device seed table allocation failed");
+        c->gpu_ready = true;
+        free_device(c);
+        return -1;
+    }
+    const double thr = (c->eps > 1e-280) ? c->eps : 1e-280;
+    launch_seed_tables(&d, L, mmax, mres, nl2, thr, nullptr);
+    HIP_OK_RET(hipGetLastError(), -1);
+    HIP_OK_RET(hipDeviceSynchronize(), -1);
+    c->gpu_ready = true;
+    return dev;
+}
+
+extern "C" int cushtns_init_gpu(shtns_cfg c) {
+    clear_error();
+    if (c->ct == nullptr) {
+        set_error(1, "cushtns_init_gpu: grid not set");
+        return -1;
+    }
+    if (c->gpu_ready) return c->device;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        set_error(3, "no GPU device available");
+        return -1;
+    }
+    return device_init(c);
+}
+
+extern "C" void cushtns_set_streams(shtns_cfg c, hipStream_t compute_stream, hipStream_t transfer_stream) {
+    if (c->own_streams && c->comp_stream) (void)hipStreamDestroy(c->comp_stream);
+    c->own_streams = false;
+    c->comp_stream = compute_stream;
+    c->xfer_stream = transfer_stream;
+}
+
+extern "C" shtns_cfg cushtns_clone(shtns_cfg c, hipStream_t compute_stream, hipStream_t transfer_stream) {
+    clear_error();
+    if (!c->gpu_ready) {
+        set_error(3, "cushtns_clone: source config has no GPU tables");
+        return nullptr;
+    }
+    shtns_info *k = new shtns_info(*c);
+    k->clone_of = (c->clone_of) ? c->clone_of : c;
+    k->ct = new double[2 * (size_t)c->nlat];
+    memcpy(k->ct, c->ct, sizeof(double) * 2 * c->nlat);
+    k->st = k->ct + c->nlat;
+    k->scratch = nullptr;
+    k->scratch_bytes = 0;
+    k->own_streams = false;
+    k->comp_stream = compute_stream;
+    k->xfer_stream = transfer_stream;
+    std::lock_guard<std::mutex> g(g_mutex);
+    k->next = g_cfg_list;
+    g_cfg_list = k;
+    return k;
+}
+
+// ------------------------------------------------------------------------------------------
+// transforms
+// ------------------------------------------------------------------------------------------
+static bool ready(shtns_cfg c, const char *who) {
+    if (c == nullptr || c->ct == nullptr) {
+        set_error(1, "%s: grid not set", who);
+        return false;
+    }
+    if (!c->gpu_ready) {
+        if (cushtns_init_gpu(c) < 0) {
+            set_error(3, "%s: no usable GPU (this library has no CPU transform path)", who);
+            return false;
+        }
+    }
+    (void)hipSetDevice(c->device);
+    return true;
+}
+
+static void *scratch(shtns_cfg c, size_t bytes) {
+    if (bytes <= c->scratch_bytes) return c->scratch;
+    if (c->scratch) {
+        (void)hipStreamSynchronize(c->comp_stream);
+        (void)hipFree(c->scratch);
+    }
+    c->scratch = nullptr;
+    c->scratch_bytes = 0;
+    if (hipMalloc(&c->scratch, bytes) != hipSuccess) {
+        set_error(2, "scratch allocation of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    c->scratch_bytes = bytes;
+    return c->scratch;
+}
+
+static void spat_strides(shtns_cfg c, long &sp, long &st) {
+    if (c->layout & SHT_PHI_CONTIGUOUS) { sp = 1; st = c->nphi; }
+    else { sp = c->nlat; st = 1; }
+}
+
+static LegendreArgs leg_args(shtns_cfg c, int ltr, int imlim, int nfields) {
+    LegendreArgs a;
+    a.t = &c->d;
+    a.lmax = c->lmax;
+    a.mmax = c->mmax;
+    a.mres = c->mres;
+    a.nlat = c->nlat;
+    a.nlat_2 = c->nlat_2;
+    a.ltr = ltr;
+    a.imlim = imlim;
+    a.nfields = nfields;
+    a.vector_mode = 0;
+    a.mpos_scale = 1.0;
+    a.lat_weight = 0;
+    return a;
+}
+
+static int clamp_ltr(shtns_cfg c, int ltr) { return (ltr < 0 || ltr > c->lmax) ? c->lmax : ltr; }
+static int imlim_of(shtns_cfg c, int ltr) {
+    const int im = ltr / c->mres;
+    return im < c->mmax ? im : c->mmax;
+}
+
+// ---- optional kernel timing (HIP events recorded around each launch on the compute stream)
+enum { PK_LEG_SYNTH = 0, PK_LEG_ANAL = 1, PK_FFT_SYNTH = 2, PK_FFT_ANAL = 3, PK_VEC = 4, PK_N = 5 };
+struct ProfScope {
+    shtns_cfg c;
+    int k;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(shtns_cfg c_, int k_) : c(c_), k(k_) {
+        if (!c->prof) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
+        (void)hipEventRecord(a, c->comp_stream);
+    }
+    ~ProfScope() {
+        if (!a) return;
+        (void)hipEventRecord(b, c->comp_stream);
+        c->prof_ev[k].emplace_back(a, b);
+    }
+};
+
+static void prof_clear(shtns_cfg c) {
+    for (auto &v : c->prof_ev) {
+        for (auto &e : v) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        v.clear();
+    }
+}
+
+// Additive (no SHTns counterpart): enable/disable per-kernel timing; clears recorded events.
+extern "C" void shtns_profile(shtns_cfg c, int enable) {
+    prof_clear(c);
+    c->prof = (enable != 0);
+}
+
+// Total milliseconds and launch counts per kernel kind since shtns_profile(c, 1):
+// [0] Legendre synthesis, [1] Legendre analysis, [2] FFT synthesis, [3] FFT analysis,
+// [4] vector conversions.  Waits for the recorded events.  Returns the number of kinds.
+extern "C" int shtns_profile_get(shtns_cfg c, double *ms, int *count) {
+    for (int k = 0; k < PK_N; ++k) {
+        double t = 0.0;
+        for (auto &e : c->prof_ev[k]) {
+            float x = 0.f;
+            (void)hipEventSynchronize(e.second);
+            if (hipEventElapsedTime(&x, e.first, e.second) == hipSuccess) t += x;
+        }
+        ms[k] = t;
+        count[k] = (int)c->prof_ev[k].size();
+    }
+    return PK_N;
+}
+
+// synthesis core: Q (nfields, spec_dist) -> V (nfields, spat_dist)
+static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
+                       double mpos_scale, int lat_weight) {
+    if (nfields <= 0) return;
+    ltr = clamp_ltr(c, ltr);
+    const int imlim = imlim_of(c, ltr);
+    const long four_dist = (long)(c->mmax + 1) * c->nlat;
+    double2 *F = (double2 *)scratch(c, (size_t)nfields * four_dist * sizeof(double2));
+    if (!F) return;
+    LegendreArgs a = leg_args(c, ltr, imlim, nfields);
+    a.mpos_scale = mpos_scale;
+    a.lat_weight = lat_weight;
+    {
+        ProfScope ps(c, PK_LEG_SYNTH);
+        launch_legendre_synth(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, F, four_dist, c->comp_stream);
+    }
+    long sp, st;
+    spat_strides(c, sp, st);
+    {
+        ProfScope ps(c, PK_FFT_SYNTH);
+        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st,
+                         c->comp_stream);
+    }
+    HIP_OK(hipGetLastError());
+}
+
+static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
+                      double mpos_scale, int lat_weight) {
+    if (nfields <= 0) return;
+    ltr = clamp_ltr(c, ltr);
+    const int imlim = imlim_of(c, ltr);
+    const long four_dist = (long)(c->mmax + 1) * c->nlat;
+    double2 *F = (double2 *)scratch(c, (size_t)nfields * four_dist * sizeof(double2));
+    if (!F) return;
+    long sp, st;
+    spat_strides(c, sp, st);
+    {
+        ProfScope ps(c, PK_FFT_ANAL);
+        launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st,
+                        c->comp_stream);
+    }
+    LegendreArgs a = leg_args(c, ltr, imlim, nfields);
+    a.mpos_scale = mpos_scale;
+    a.lat_weight = lat_weight;
+    {
+        ProfScope ps(c, PK_LEG_ANAL);
+        launch_legendre_anal(a, F, four_dist, reinterpret_cast<double2 *>(Q), spec_dist, c->lmax, c->comp_stream);
+    }
+    HIP_OK(hipGetLastError());
+}
+
+static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, double *Vp, int ltr, int nfields,
+                        long spec_dist, long spat_dist) {
+    if (nfields <= 0) return;
+    ltr = clamp_ltr(c, ltr);
+    const int imlim = imlim_of(c, ltr);
+    const long four_dist = (long)(c->mmax + 1) * c->nlat;
+    const long vw_dist = nlm_calc(c->lmax + 1, c->mmax, c->mres) + 2;
+    const size_t fbytes = (size_t)2 * nfields * four_dist * sizeof(double2);
+    const size_t vbytes = (size_t)2 * nfields * vw_dist * sizeof(double2);
+    char *base = (char *)scratch(c, fbytes + vbytes);
+    if (!base) return;
+    double2 *F = (double2 *)base;
+    double2 *VW = (double2 *)(base + fbytes);
+    launch_sphtor_to_vw(reinterpret_cast<const double2 *>(S), reinterpret_cast<const double2 *>(T), spec_dist, VW,
+                        vw_dist, c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);
+    LegendreArgs a = leg_args(c, ltr + 1, imlim, 2 * nfields);
+    a.vector_mode = 1;
+    launch_legendre_synth(a, VW, vw_dist, c->lmax + 1, F, four_dist, c->comp_stream);
+    long sp, st;
+    spat_strides(c, sp, st);
+    launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, 2 * four_dist, Vt, spat_dist, sp, st,
+                     c->comp_stream);
+    launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F + four_dist, 2 * four_dist, Vp, spat_dist, sp,
+                     st, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+
+static void vanal_core(shtns_cfg c, const double *Vt, const double *Vp, cplx *S, cplx *T, int ltr, int nfields,
+                       long spat_dist, long spec_dist) {
+    if (nfields <= 0) return;
+    ltr = clamp_ltr(c, ltr);
+    const int imlim = imlim_of(c, ltr);
+    const long four_dist = (long)(c->mmax + 1) * c->nlat;
+    const long vw_dist = nlm_calc(c->lmax + 1, c->mmax, c->mres) + 2;
+    const size_t fbytes = (size_t)2 * nfields * four_dist * sizeof(double2);
+    const size_t vbytes = (size_t)2 * nfields * vw_dist * sizeof(double2);
+    char *base = (char *)scratch(c, fbytes + vbytes);
+    if (!base) return;
+    double2 *F = (double2 *)base;
+    double2 *VW = (double2 *)(base + fbytes);
+    long sp, st;
+    spat_strides(c, sp, st);
+    launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, Vt, spat_dist, F, 2 * four_dist, sp, st,
+                    c->comp_stream);
+    launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, Vp, spat_dist, F + four_dist, 2 * four_dist, sp,
+                    st, c->comp_stream);
+    LegendreArgs a = leg_args(c, ltr + 1, imlim, 2 * nfields);
+    a.vector_mode = 1;
+    a.lat_weight = 1;
+    launch_legendre_anal(a, F, four_dist, VW, vw_dist, c->lmax + 1, c->comp_stream);
+    launch_vw_to_sphtor(VW, vw_dist, reinterpret_cast<double2 *>(S), reinterpret_cast<double2 *>(T), spec_dist,
+                        c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+
+// ---- device-pointer API ----
+extern "C" void cu_SH_to_spat_batch(shtns_cfg c, cplx *Q, double *V, int ltr, int nf, long spec_dist, long spat_dist) {
+    clear_error();
+    if (!ready(c, "cu_SH_to_spat")) return;
+    synth_core(c, Q, V, ltr, nf, spec_dist, spat_dist, 1.0, 0);
+}
+extern "C" void cu_spat_to_SH_batch(shtns_cfg c, double *V, cplx *Q, int ltr, int nf, long spat_dist, long spec_dist) {
+    clear_error();
+    if (!ready(c, "cu_spat_to_SH")) return;
+    anal_core(c, V, Q, ltr, nf, spat_dist, spec_dist, 1.0, 1);
+}
+extern "C" void cu_SH_to_spat(shtns_cfg c, cplx *Q, double *V, int ltr) {
+    cu_SH_to_spat_batch(c, Q, V, ltr, 1, c->nlm, c->nspat);
+}
+extern "C" void cu_spat_to_SH(shtns_cfg c, double *V, cplx *Q, int ltr) {
+    cu_spat_to_SH_batch(c, V, Q, ltr, 1, c->nspat, c->nlm);
+}
+extern "C" void cu_SHsphtor_to_spat_batch(shtns_cfg c, cplx *S, cplx *T, double *Vt, double *Vp, int ltr, int nf,
+                                          long spec_dist, long spat_dist) {
+    clear_error();
+    if (!ready(c, "cu_SHsphtor_to_spat")) return;
+    vsynth_core(c, S, T, Vt, Vp, ltr, nf, spec_dist, spat_dist);
+}
+extern "C" void cu_spat_to_SHsphtor_batch(shtns_cfg c, double *Vt, double *Vp, cplx *S, cplx *T, int ltr, int nf,
+                                          long spat_dist, long spec_dist) {
+    clear_error();
+    if (!ready(c, "cu_spat_to_SHsphtor")) return;
+    vanal_core(c, Vt, Vp, S, T, ltr, nf, spat_dist, spec_dist);
+}
+extern "C" void cu_SHsphtor_to_spat(shtns_cfg c, cplx *S, cplx *T, double *Vt, double *Vp, int ltr) {
+    cu_SHsphtor_to_spat_batch(c, S, T, Vt, Vp, ltr, 1, c->nlm, c->nspat);
+}
+extern "C" void cu_spat_to_SHsphtor(shtns_cfg c, double *Vt, double *Vp, cplx *S, cplx *T, int ltr) {
+    cu_spat_to_SHsphtor_batch(c, Vt, Vp, S, T, ltr, 1, c->nspat, c->nlm);
+}
+extern "C" void cu_SH_to_spat_adjoint_batch(shtns_cfg c, double *V, cplx *Q, int ltr, int nf, long spat_dist,
+                                            long spec_dist) {
+    clear_error();
+    if (!ready(c, "cu_SH_to_spat_adjoint")) return;
+    anal_core(c, V, Q, ltr, nf, spat_dist, spec_dist, 2.0, 0);
+}
+extern "C" void cu_spat_to_SH_adjoint_batch(shtns_cfg c, cplx *Q, double *V, int ltr, int nf, long spec_dist,
+                                            long spat_dist) {
+    clear_error();
+    if (!ready(c, "cu_spat_to_SH_adjoint")) return;
+    synth_core(c, Q, V, ltr, nf, spec_dist, spat_dist, 0.5, 1);
+}
+
+// ---- host-pointer API: copy in, run on the GPU, copy out (synchronous) ----
+struct HostStage {
+    shtns_cfg c;
+    std::vector<void *> bufs;
+    ~HostStage() {
+        for (void *p : bufs) (void)hipFree(p);
+    }
+    void *dev(size_t bytes) {
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        bufs.push_back(p);
+        return p;
+    }
+};
+
+static bool h2d(void *d, const void *h, size_t n, hipStream_t s) {
+    return hipMemcpyAsync(d, h, n, hipMemcpyHostToDevice, s) == hipSuccess;
+}
+static bool d2h(void *h, const void *d, size_t n, hipStream_t s) {
+    return hipMemcpyAsync(h, d, n, hipMemcpyDeviceToHost, s) == hipSuccess;
+}
+
+extern "C" void SH_to_spat_l(shtns_cfg c, cplx *Qlm, double *Vr, int ltr) {
+    clear_error();
+    if (!ready(c, "SH_to_spat")) return;
+    HostStage st{c, {}};
+    const size_t qb = sizeof(cplx) * c->nlm, vb = sizeof(double) * c->nspat;
+    cplx *dq = (cplx *)st.dev(qb);
+    double *dv = (double *)st.dev(vb);
+    if (!dq || !dv) { set_error(2, "SH_to_spat: device allocation failed"); return; }
+    if (!h2d(dq, Qlm, qb, c->comp_stream)) { set_error(2, "SH_to_spat: copy failed"); return; }
+    synth_core(c, dq, dv, ltr, 1, c->nlm, c->nspat, 1.0, 0);
+    if (g_last_err) return;
+    if (!d2h(Vr, dv, vb, c->comp_stream)) { set_error(2, "SH_to_spat: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(c->comp_stream));
+}
+
+extern "C" void spat_to_SH_l(shtns_cfg c, double *Vr, cplx *Qlm, int ltr) {
+    clear_error();
+    if (!ready(c, "spat_to_SH")) return;
+    HostStage st{c, {}};
+    const size_t qb = sizeof(cplx) * c->nlm, vb = sizeof(double) * c->nspat;
+    cplx *dq = (cplx *)st.dev(qb);
+    double *dv = (double *)st.dev(vb);
+    if (!dq || !dv) { set_error(2, "spat_to_SH: device allocation failed"); return; }
+    if (!h2d(dv, Vr, vb, c->comp_stream)) { set_error(2, "spat_to_SH: copy failed"); return; }
+    anal_core(c, dv, dq, ltr, 1, c->nspat, c->nlm, 1.0, 1);
+    if (g_last_err) return;
+    if (!d2h(Qlm, dq, qb, c->comp_stream)) { set_error(2, "spat_to_SH: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(c->comp_stream));
+}
+
+extern "C" void SHsphtor_to_spat_l(shtns_cfg c, cplx *S, cplx *T, double *Vt, double *Vp, int ltr) {
+    clear_error();
+    if (!ready(c, "SHsphtor_to_spat")) return;
+    HostStage st{c, {}};
+    const size_t qb = sizeof(cplx) * c->nlm, vb = sizeof(double) * c->nspat;
+    cplx *ds = (cplx *)st.dev(qb), *dt = (cplx *)st.dev(qb);
+    double *dvt = (double *)st.dev(vb), *dvp = (double *)st.dev(vb);
+    if (!ds || !dt || !dvt || !dvp) { set_error(2, "SHsphtor_to_spat: device allocation failed"); return; }
+    const hipStream_t s = c->comp_stream;
+    if (S) { if (!h2d(ds, S, qb, s)) return; } else HIP_OK(hipMemsetAsync(ds, 0, qb, s));
+    if (T) { if (!h2d(dt, T, qb, s)) return; } else HIP_OK(hipMemsetAsync(dt, 0, qb, s));
+    vsynth_core(c, ds, dt, dvt, dvp, ltr, 1, c->nlm, c->nspat);
+    if (g_last_err) return;
+    if (!d2h(Vt, dvt, vb, s) || !d2h(Vp, dvp, vb, s)) { set_error(2, "SHsphtor_to_spat: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+extern "C" void spat_to_SHsphtor_l(shtns_cfg c, double *Vt, double *Vp, cplx *S, cplx *T, int ltr) {
+    clear_error();
+    if (!ready(c, "spat_to_SHsphtor")) return;
+    HostStage st{c, {}};
+    const size_t qb = sizeof(cplx) * c->nlm, vb = sizeof(double) * c->nspat;
+    cplx *ds = (cplx *)st.dev(qb), *dt = (cplx *)st.dev(qb);
+    double *dvt = (double *)st.dev(vb), *dvp = (double *)st.dev(vb);
+    if (!ds || !dt || !dvt || !dvp) { set_error(2, "spat_to_SHsphtor: device allocation failed"); return; }
+    const hipStream_t s = c->comp_stream;
+    if (!h2d(dvt, Vt, vb, s) || !h2d(dvp, Vp, vb, s)) { set_error(2, "spat_to_SHsphtor: copy failed"); return; }
+    vanal_core(c, dvt, dvp, ds, dt, ltr, 1, c->nspat, c->nlm);
+    if (g_last_err) return;
+    if (!d2h(S, ds, qb, s) || !d2h(T, dt, qb, s)) { set_error(2, "spat_to_SHsphtor: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+extern "C" void SH_to_spat(shtns_cfg c, cplx *Qlm, double *Vr) { SH_to_spat_l(c, Qlm, Vr, c->lmax); }
+extern "C" void spat_to_SH(shtns_cfg c, double *Vr, cplx *Qlm) { spat_to_SH_l(c, Vr, Qlm, c->lmax); }
+extern "C" void SHsphtor_to_spat(shtns_cfg c, cplx *S, cplx *T, double *Vt, double *Vp) {
+    SHsphtor_to_spat_l(c, S, T, Vt, Vp, c->lmax);
+}
+extern "C" void spat_to_SHsphtor(shtns_cfg c, double *Vt, double *Vp, cplx *S, cplx *T) {
+    spat_to_SHsphtor_l(c, Vt, Vp, S, T, c->lmax);
+}
+extern "C" void SHsph_to_spat_l(shtns_cfg c, cplx *S, double *Vt, double *Vp, int ltr) {
+    SHsphtor_to_spat_l(c, S, nullptr, Vt, Vp, ltr);
+}
+extern "C" void SHtor_to_spat_l(shtns_cfg c, cplx *T, double *Vt, double *Vp, int ltr) {
+    SHsphtor_to_spat_l(c, nullptr, T, Vt, Vp, ltr);
+}
+extern "C" void SHsph_to_spat(shtns_cfg c, cplx *S, double *Vt, double *Vp) { SHsph_to_spat_l(c, S, Vt, Vp, c->lmax); }
+extern "C" void SHtor_to_spat(shtns_cfg c, cplx *T, double *Vt, double *Vp) { SHtor_to_spat_l(c, T, Vt, Vp, c->lmax); }

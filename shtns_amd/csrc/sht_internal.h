@@ -1,0 +1,115 @@
+// sht_internal.h -- private configuration and kernel-launch interface of shtns_amd.
+//
+// The configuration begins with the exact member layout of SHTns's public struct
+// (shtns.h:79-94) so callers that read cfg->nlm, cfg->ct, ... through the public header
+// keep working; everything after it is private to this library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#define SHTNS_PRIVATE
+#include "../../include/shtns.h"
+#include "../../include/shtns_hip.h"
+
+namespace shtns_amd {
+
+// Kernel tile constants shared by host and device code.
+constexpr int kColsPerTile = 16;        // MFMA N: 4 fields x (E.re, E.im, O.re, O.im)
+constexpr int kFieldsPerTile = 4;
+constexpr int kStepBlock = 16;          // recurrence steps per analysis tile (MFMA M)
+
+// Mixed-radix plan of the M = nphi/2 point complex FFT (stage s has radix radix[s]).
+struct FftPlan {
+    int M;
+    int nstages;
+    int radix[24];
+};
+
+// Device tables built once per grid (see build_device_tables in sht_host.cpp).
+struct DeviceTables {
+    // per latitude pair k < nlat_2
+    double *x = nullptr;       // cos(theta_k)
+    double *x2 = nullptr;      // cos^2
+    double *st = nullptr;      // sin(theta_k)
+    double *wq = nullptr;      // Gauss weight w_k * 2pi/nphi (both hemispheres share it)
+    // per order im: offsets into the per-step coefficient arrays
+    int *step_ofs = nullptr;   // [mmax+2] start of order im in alpha/beta/pre-mix arrays
+    double2 *ab = nullptr;     // [nsteps] (alpha_i, beta_i): z_{i+1} = (alpha x^2 + beta) z_i + z_{i-1}
+    double4 *pm = nullptr;     // [nsteps] pre/post-mix (ue_i, ue1_i, uo_i, 0):
+                               //   P_{m+2i} = ue_i z_i + ue1_{i-1} z_{i-1},  P_{m+2i+1} = x uo_i z_i
+    // per (order, latitude pair): recurrence start (scalar functions, and /sin for vectors)
+    int *i0 = nullptr;         // [(mmax+2) * nlat_2] first step with significant values
+    double2 *z0 = nullptr;     // [(mmax+2) * nlat_2] (z_{i0-1}, z_{i0})
+    // FFT: roots e^{2 pi i k/nphi}, k < nphi
+    double2 *roots = nullptr;
+    int *fft_pos = nullptr;    // [M] position of output bin k after the in-place DIF FFT
+    FftPlan plan{};
+    // vector transforms
+};
+
+}  // namespace shtns_amd
+
+struct shtns_info {
+    // ---- public prefix: identical to SHTns's shtns.h:79-94 ----
+    unsigned int nlm;
+    unsigned short lmax, mmax, mres, nlat_2;
+    unsigned int nlat, nphi, nspat;
+    unsigned short *li, *mi;
+    double *ct, *st;
+    unsigned int nlat_padded, nlm_cplx;
+    // ---- private ----
+    int norm;
+    int layout;                // SHT_THETA_CONTIGUOUS (native) or SHT_PHI_CONTIGUOUS
+    double eps;                // polar threshold given to shtns_set_grid
+    std::vector<double> wg;    // Gauss weights (raw, sum 2), nlat_2 values (north half)
+    int nsteps_total = 0;      // coefficient slots over all orders (scalar and vector use)
+    int device = -1;
+    bool gpu_ready = false;
+    hipStream_t comp_stream = nullptr;
+    hipStream_t xfer_stream = nullptr;
+    bool own_streams = false;
+    shtns_amd::DeviceTables d;
+    // scratch: Fourier buffers, vector conversion buffers (grown on demand)
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    shtns_info *next = nullptr;
+    shtns_info *clone_of = nullptr;   // tables borrowed from this config
+    // optional per-kernel timing with HIP events on the compute stream (shtns_profile)
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[8];
+};
+
+namespace shtns_amd {
+
+// kernel launchers (sht_kernels.hip)
+struct LegendreArgs {
+    const DeviceTables *t;
+    int lmax, mmax, mres, nlat, nlat_2, ltr, imlim;
+    int nfields;
+    int vector_mode;          // 0: scalar functions, 1: Y/sin(theta) (spin-1 V,W spectra)
+    double mpos_scale;        // factor on m>0 orders (adjoints use 2 or 0.5)
+    int lat_weight;           // synthesis: multiply output by wq_k; analysis: 0 => unit weights
+};
+
+// Fourier buffer layout: F[field][im][nlat] complex (north to south).
+void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
+                           double2 *F, long four_dist, hipStream_t s);
+void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dist,
+                          double2 *Qlm, long spec_dist, int spec_ltop, hipStream_t s);
+// Spatial element (ip, it) of a field sits at ip*sp + it*st (theta-contiguous: sp=nlat, st=1).
+void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
+                      const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s);
+void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
+                     const double *V, long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s);
+void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s);
+void launch_sphtor_to_vw(const double2 *S, const double2 *T, long spec_dist, double2 *VW, long vw_dist,
+                         int lmax, int mmax, int mres, int ltr, int imlim, int nfields, hipStream_t s);
+void launch_vw_to_sphtor(const double2 *VW, long vw_dist, double2 *S, double2 *T, long spec_dist,
+                         int lmax, int mmax, int mres, int ltr, int imlim, int nfields, hipStream_t s);
+
+}  // namespace shtns_amd
