@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <cstdlib>
 
 #include "sht_internal.h"
 
@@ -140,13 +141,21 @@ __device__ inline OrderInfo order_info(int im, int lmax_spec, int mmax, int mres
 
 // ---------------------------------------------------------------------------------------
 // Synthesis: F[f][im][k] = sum_l Q_lm P_lm(x_k)   (both hemispheres)
-// Block = 4 waves; wave w handles latitude tile (blockIdx.z*4 + w) of 64 latitude pairs,
-// for order im = blockIdx.x and fields 4*blockIdx.y .. +3.
-// Per 4-step block: lanes compute z_i for their latitude (4 steps) -> LDS; MFMA
-// D[16 lat][16 col] += Y[16 lat][4 step] * B[4 step][16 col] for the 4 row tiles.
+// Block = 4 waves for one order im, 4*NCG fields (blockIdx.y) and 4 latitude tiles of 64
+// pairs (blockIdx.z); wave w owns tile blockIdx.z*4 + w, lane = latitude pair.
+//   * the block pre-mixes the coefficients of 32 steps at a time into LDS in the layout of
+//     the MFMA B operand: qs[step][col], col = 16*cg + 4*field + (E.re, E.im, O.re, O.im),
+//     E_i = ue_i Q_{m+2i} + ue1_i Q_{m+2i+2},  O_i = uo_i Q_{m+2i+1};
+//   * per 4 steps each lane advances its recurrence (z_i for its latitude) into a wave-private
+//     LDS tile; the tile is read back as the A operand (16 latitudes x 4 steps) and
+//     D[16 lat][16 col] += A * B runs for 4 row tiles x NCG column groups on the matrix cores.
+//   * the accumulators go straight to F: lanes holding E and O of one field are lane^2
+//     partners (north = E + x O, south = E - x O).
 // ---------------------------------------------------------------------------------------
 constexpr int kSynthWaves = 4;
+constexpr int kSynthChunk = 32;   // steps pre-mixed per LDS fill
 
+template <int NCG>
 __global__ __launch_bounds__(256) void legendre_synth_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
@@ -154,137 +163,153 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     const double2 *__restrict__ Q, long spec_dist, int lmax_spec, double2 *__restrict__ F, long four_dist,
     int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
     double mpos_scale, int lat_weight) {
+    constexpr int NCOL = 16 * NCG;
+    __shared__ double qs[kSynthChunk][NCOL + 1];
     __shared__ double ytile[kSynthWaves][4][64];
-    __shared__ double dout[kSynthWaves][64][17];
+    __shared__ int s_red[kSynthWaves];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int im = blockIdx.x;
-    const int fg = blockIdx.y;
+    const int fbase = blockIdx.y * 4 * NCG;
     const int tile = blockIdx.z * kSynthWaves + wave;
-    const int k = tile * 64 + lane;              // latitude pair owned by this lane
-    if (tile * 64 >= nlat_2) return;             // whole wave idle (no barriers below)
+    const int k = tile * 64 + lane;   // latitude pair owned by this lane
     const bool kvalid = (k < nlat_2);
     const int kk = kvalid ? k : (nlat_2 - 1);
 
-    // column handled by this lane in the B operand / D columns
-    const int col = lane & 15;
-    const int fcol = fg * 4 + (col >> 2);
-    const int comp = col & 3;
-    const bool fvalid = (fcol < nfields);
+    double4_t acc[4][NCG];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) acc[r][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
 
-    const double x = xk[kk];
-    double4_t acc[4];
-    for (int r = 0; r < 4; ++r) acc[r] = (double4_t){0.0, 0.0, 0.0, 0.0};
-
-    if (im <= imlim) {
-        const OrderInfo o = order_info(im, lmax_spec, mmax, mres, ltr, vector_mode);
+    const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, vector_mode);
+    const int nsteps = (im <= imlim) ? o.nsteps : 0;
+    if (nsteps > 0) {
         const int sofs = step_ofs[o.slot];
         const double2 *abm = ab + sofs;
         const double4 *pmm = pm + sofs;
         const double x2 = x2k[kk];
-        // recurrence start for this lane
-        int i0 = kvalid ? i0tab[(size_t)o.slot * nlat_2 + kk] : o.nsteps;
+        int i0 = kvalid ? i0tab[(size_t)o.slot * nlat_2 + kk] : nsteps;
         double2 sd = z0tab[(size_t)o.slot * nlat_2 + kk];
         if (o.sinv) {
             const double inv = 1.0 / stk[kk];
             sd.x *= inv;
             sd.y *= inv;
         }
-        if (i0 > o.nsteps) i0 = o.nsteps;
-        // wave-uniform range of steps
-        int ibeg = i0;
-        for (int off = 32; off >= 1; off >>= 1) ibeg = min(ibeg, __shfl_xor(ibeg, off));
-        int imax0 = i0;
-        for (int off = 32; off >= 1; off >>= 1) imax0 = max(imax0, __shfl_xor(imax0, off));
+        if (i0 > nsteps) i0 = nsteps;
+        // block-uniform first step and wave-uniform last seed step
+        int wmin = i0, wmax = i0;
+        for (int off = 32; off >= 1; off >>= 1) {
+            wmin = min(wmin, __shfl_xor(wmin, off));
+            wmax = max(wmax, __shfl_xor(wmax, off));
+        }
+        if (lane == 0) s_red[wave] = wmin;
+        __syncthreads();
+        int ibeg = nsteps;
+        for (int w = 0; w < kSynthWaves; ++w) ibeg = min(ibeg, s_red[w]);
         ibeg &= ~3;
         double za = 0.0, zb = 0.0;
         if (i0 == ibeg) { za = sd.x; zb = sd.y; }
-
-        const double2 *Qf = Q + (size_t)(fvalid ? fcol : 0) * spec_dist + o.spec_base;
         const double qscale = (im == 0) ? 1.0 : mpos_scale;
-        const int ltop = ltr;
-        // B operand for step i (this lane: step i + lane/16, column col)
-        auto load_b = [&](int i) -> double {
-            const int ii = i + (lane >> 4);
-            if (!fvalid || ii >= o.nsteps) return 0.0;
-            const int le = o.mfun + 2 * ii;
-            const double4 c = pmm[ii];
-            double v;
-            if (comp < 2) {
-                const double2 q0 = Qf[le];
-                double a = (comp == 0) ? q0.x : q0.y;
-                double b = 0.0;
-                if (le + 2 <= ltop) {
-                    const double2 q2 = Qf[le + 2];
-                    b = (comp == 0) ? q2.x : q2.y;
-                }
-                v = c.x * a + c.y * b;
-            } else {
-                double a = 0.0;
-                if (le + 1 <= ltop) {
-                    const double2 q1 = Qf[le + 1];
-                    a = (comp == 2) ? q1.x : q1.y;
-                }
-                v = c.z * a;
-            }
-            if (o.real_only && (comp & 1)) v = 0.0;
-            return v * qscale;
-        };
 
-        double bnext = load_b(ibeg);
-        for (int i = ibeg; i < o.nsteps; i += 4) {
-            const double bcur = bnext;
-            // 4 recurrence steps for this lane's latitude
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int ii = i + s;
-                if (ii < imax0) {
-                    if (ii == i0) { za = sd.x; zb = sd.y; }
+        for (int c0 = ibeg; c0 < nsteps; c0 += kSynthChunk) {
+            __syncthreads();   // previous chunk's B operands consumed
+            // pre-mix steps c0 .. c0+31 into qs
+            for (int e = threadIdx.x; e < kSynthChunk * NCOL; e += 256) {
+                const int sl = e / NCOL, col = e - sl * NCOL;
+                const int ii = c0 + sl;
+                const int f = fbase + (col >> 2);
+                const int comp = col & 3;
+                double v = 0.0;
+                if (ii < nsteps && f < nfields && !(o.real_only && (comp & 1))) {
+                    const double2 *Qf = Q + (size_t)f * spec_dist + o.spec_base;
+                    const int le = o.mfun + 2 * ii;
+                    const double4 cf = pmm[ii];
+                    if (comp < 2) {
+                        const double2 q0 = Qf[le];
+                        double a = (comp == 0) ? q0.x : q0.y, b = 0.0;
+                        if (le + 2 <= ltr) {
+                            const double2 q2 = Qf[le + 2];
+                            b = (comp == 0) ? q2.x : q2.y;
+                        }
+                        v = cf.x * a + cf.y * b;
+                    } else if (le + 1 <= ltr) {
+                        const double2 q1 = Qf[le + 1];
+                        v = cf.z * ((comp == 2) ? q1.x : q1.y);
+                    }
+                    v *= qscale;
                 }
-                ytile[wave][s][lane] = (ii >= i0) ? zb : 0.0;
-                const double2 c = abm[min(ii, o.nsteps - 1)];
-                const double zn = fma(fma(c.x, x2, c.y), zb, za);
-                za = zb;
-                zb = zn;
+                qs[sl][col] = v;
             }
-            bnext = load_b(i + 4);
-            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS writes visible within the wave
-            __builtin_amdgcn_wave_barrier();
+            __syncthreads();
+            const int cend = min(c0 + kSynthChunk, nsteps);
+            for (int i = c0; i < cend; i += 4) {
+                if (i <= wmax) {   // some lane of this wave starts in this 4-step block
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const double a = ytile[wave][lane >> 4][r * 16 + (lane & 15)];
-                acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bcur, acc[r], 0, 0, 0);
+                    for (int s = 0; s < 4; ++s) {
+                        const int ii = i + s;
+                        if (ii == i0) { za = sd.x; zb = sd.y; }
+                        ytile[wave][s][lane] = zb;
+                        const double2 cf = abm[min(ii, nsteps - 1)];
+                        const double zn = fma(fma(cf.x, x2, cf.y), zb, za);
+                        za = zb;
+                        zb = zn;
+                    }
+                } else {
+#pragma unroll
+                    for (int s = 0; s < 4; ++s) {
+                        ytile[wave][s][lane] = zb;
+                        const double2 cf = abm[min(i + s, nsteps - 1)];
+                        const double zn = fma(fma(cf.x, x2, cf.y), zb, za);
+                        za = zb;
+                        zb = zn;
+                    }
+                }
+                __builtin_amdgcn_s_waitcnt(0xc07f);
+                __builtin_amdgcn_wave_barrier();
+                double a[4];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) a[r] = ytile[wave][lane >> 4][r * 16 + (lane & 15)];
+                const int sl = i - c0 + (lane >> 4);
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) {
+                    const double b = qs[sl][c * 16 + (lane & 15)];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][c], 0, 0, 0);
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            __builtin_amdgcn_wave_barrier();
         }
     }
 
     // gfx950 D layout of v_mfma_f64_16x16x4f64: register j of lane l holds
-    // D[row = 4*j + l/16][col = l%16]  (row tile r covers latitudes 16r..16r+15).
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) dout[wave][r * 16 + 4 * j + (lane >> 4)][lane & 15] = acc[r][j];
-    __builtin_amdgcn_s_waitcnt(0xc07f);
-    __builtin_amdgcn_wave_barrier();
-
-    if (!kvalid) return;
-    const double wmul = lat_weight ? wqk[k] : 1.0;
+    // D[row = 4*j + l/16][col = l%16]; row tile r covers latitudes tile*64 + 16r + row.
+    const int col = lane & 15;
+    const int comp = col & 3;
     const bool real_only = (im == 0);
-    for (int f4 = 0; f4 < 4; ++f4) {
-        const int f = fg * 4 + f4;
-        if (f >= nfields) break;
-        const double er = dout[wave][lane][4 * f4 + 0];
-        const double ei = dout[wave][lane][4 * f4 + 1];
-        const double orr = dout[wave][lane][4 * f4 + 2] * x;
-        const double oi = dout[wave][lane][4 * f4 + 3] * x;
-        double2 n = make_double2((er + orr) * wmul, (ei + oi) * wmul);
-        double2 s = make_double2((er - orr) * wmul, (ei - oi) * wmul);
-        if (real_only) { n.y = 0.0; s.y = 0.0; }
-        double2 *Ff = F + (size_t)f * four_dist + (size_t)im * nlat;
-        Ff[k] = n;
-        Ff[nlat - 1 - k] = s;
+#pragma unroll
+    for (int c = 0; c < NCG; ++c) {
+        const int f = fbase + c * 4 + (col >> 2);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double mine = acc[r][c][j];
+                const double other = __shfl_xor(mine, 2);   // E <-> O partner
+                const int kr = tile * 64 + r * 16 + 4 * j + (lane >> 4);
+                if (comp < 2 && kr < nlat_2 && f < nfields) {
+                    const double x = xk[kr];
+                    const double wmul = lat_weight ? wqk[kr] : 1.0;
+                    double n = (mine + x * other) * wmul;
+                    double sth = (mine - x * other) * wmul;
+                    if (real_only && comp == 1) { n = 0.0; sth = 0.0; }
+                    double *Ff = reinterpret_cast<double *>(F + (size_t)f * four_dist + (size_t)im * nlat);
+                    Ff[2 * kr + comp] = n;
+                    Ff[2 * (nlat - 1 - kr) + comp] = sth;
+                }
+            }
+        }
     }
 }
 
@@ -292,22 +317,41 @@ void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_
                            double2 *F, long four_dist, hipStream_t s) {
     const DeviceTables *t = a.t;
     const int ntiles = (a.nlat_2 + 63) / 64;
-    dim3 grid(a.mmax + 1, (a.nfields + 3) / 4, (ntiles + kSynthWaves - 1) / kSynthWaves);
-    hipLaunchKernelGGL(legendre_synth_kernel, grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq,
-                       t->step_ofs, t->ab, t->pm, t->i0, t->z0, Qlm, spec_dist, spec_ltop, F, four_dist, a.mmax,
-                       a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale,
-                       a.lat_weight);
+    const int nz = (ntiles + kSynthWaves - 1) / kSynthWaves;
+    if (a.nfields > 8) {
+        dim3 grid(a.mmax + 1, (a.nfields + 15) / 16, nz);
+        hipLaunchKernelGGL((legendre_synth_kernel<4>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq,
+                           t->step_ofs, t->ab, t->pm, t->i0, t->z0, Qlm, spec_dist, spec_ltop, F, four_dist, a.mmax,
+                           a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale,
+                           a.lat_weight);
+    } else if (a.nfields > 4) {
+        dim3 grid(a.mmax + 1, 1, nz);
+        hipLaunchKernelGGL((legendre_synth_kernel<2>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq,
+                           t->step_ofs, t->ab, t->pm, t->i0, t->z0, Qlm, spec_dist, spec_ltop, F, four_dist, a.mmax,
+                           a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale,
+                           a.lat_weight);
+    } else {
+        dim3 grid(a.mmax + 1, 1, nz);
+        hipLaunchKernelGGL((legendre_synth_kernel<1>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq,
+                           t->step_ofs, t->ab, t->pm, t->i0, t->z0, Qlm, spec_dist, spec_ltop, F, four_dist, a.mmax,
+                           a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale,
+                           a.lat_weight);
+    }
 }
 
 // ---------------------------------------------------------------------------------------
 // Analysis: Q_lm = sum_k w_k P_lm(x_k) F_m(x_k)
-// Block = NW waves (all latitudes of one order, fields 4*blockIdx.y..+3); each wave owns
-// NLR latitude tiles of 64.  Per 16-step block: lanes compute z for their latitudes ->
-// LDS, MFMA D[16 step][16 col] += Y[16 step][4 lat] * E[4 lat][16 col], then the waves'
-// partial D are summed through LDS and post-mixed into Q (even: ue_i acc_i + ue1_{i-1}
-// acc_{i-1}; odd: uo_i acc_i).  Deterministic: fixed summation order, no atomics.
+// Block = NW waves for one order im and 4*NCG fields; wave w owns latitude tiles
+// w + NW*r (r < NLR) of 64 pairs, lane = latitude pair.  The weighted E/O values of the
+// block's fields for the owned latitudes are held in registers as MFMA B operands.
+// Per 16-step block: each lane advances its recurrences into a wave-private LDS tile, the
+// tile is read back as the A operand (16 steps x 4 latitudes) and
+// D[16 step][16 col] += A * B runs over all owned K-groups; the waves' partial D go to a
+// double-buffered LDS area, one barrier, then every thread sums one (step, column) over
+// the waves and applies the post-mix (even: ue_i acc_i + ue1_{i-1} acc_{i-1}; odd:
+// uo_i acc_i) and stores Q.  Fixed summation order: results are deterministic.
 // ---------------------------------------------------------------------------------------
-template <int NW, int NLR>
+template <int NW, int NLR, int NCG>
 __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
@@ -315,20 +359,16 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const double2 *__restrict__ F, long four_dist, double2 *__restrict__ Q, long spec_dist, int lmax_spec,
     int spec_ltop, int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
     double mpos_scale, int lat_weight) {
+    constexpr int NCOL = 16 * NCG;
     __shared__ double ytile[NW][16][65];
-    __shared__ double red[NW][16][17];
-    __shared__ double carry[16];
+    __shared__ double red[2][NW][16][NCOL + 1];
+    __shared__ double carry[2][NCOL];
     __shared__ int s_ibeg[NW];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int im = blockIdx.x;
-    const int fg = blockIdx.y;
-    const int col = lane & 15;
-    const int fcol = fg * 4 + (col >> 2);
-    const int comp = col & 3;
-    const bool fvalid = (fcol < nfields);
-    const int m = im * mres;
+    const int fbase = blockIdx.y * 4 * NCG;
 
     const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, vector_mode);
     const int nsteps = (im <= imlim) ? o.nsteps : 0;
@@ -336,12 +376,10 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const double2 *abm = ab + sofs;
     const double4 *pmm = pm + sofs;
 
-    // per owned latitude tile: recurrence state
     double za[NLR], zb[NLR], x2v[NLR], sdx[NLR], sdy[NLR];
     int i0v[NLR];
-    // E operand registers: Ereg[r][kg] for latitude tile r, K-group kg (4 latitudes)
-    double Ereg[NLR][16];
-    int my_ibeg = nsteps;
+    double Ereg[NLR][NCG][16];
+    int my_ibeg = nsteps, my_imax = 0;
 #pragma unroll
     for (int r = 0; r < NLR; ++r) {
         const int k = (wave + NW * r) * 64 + lane;
@@ -360,28 +398,37 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
         sdx[r] = sd.x;
         sdy[r] = sd.y;
         my_ibeg = min(my_ibeg, i0);
-        // B operand (E/O values) for K-group kg: latitude tile_base + 4*kg + lane/16, column col
+        my_imax = max(my_imax, kv ? i0 : 0);
+        // B operand for K-group kg: latitude tile_base + 4*kg + lane/16, column lane%16 of group c
 #pragma unroll
-        for (int kg = 0; kg < 16; ++kg) {
-            const int kl = (wave + NW * r) * 64 + 4 * kg + (lane >> 4);
-            double v = 0.0;
-            if (kl < nlat_2 && fvalid && im <= imlim) {
-                const double2 *Ff = F + (size_t)fcol * four_dist + (size_t)im * nlat;
-                const double2 fn = Ff[kl];
-                const double2 fs = Ff[nlat - 1 - kl];
-                const double w = lat_weight ? wqk[kl] : 1.0;
-                if (comp == 0) v = (fn.x + fs.x) * w;
-                else if (comp == 1) v = (fn.y + fs.y) * w;
-                else if (comp == 2) v = (fn.x - fs.x) * w * xk[kl];
-                else v = (fn.y - fs.y) * w * xk[kl];
-                if (o.real_only && (comp & 1)) v = 0.0;
+        for (int c = 0; c < NCG; ++c) {
+            const int col = lane & 15;
+            const int f = fbase + 4 * c + (col >> 2);
+            const int comp = col & 3;
+#pragma unroll
+            for (int kg = 0; kg < 16; ++kg) {
+                const int kl = (wave + NW * r) * 64 + 4 * kg + (lane >> 4);
+                double v = 0.0;
+                if (kl < nlat_2 && f < nfields && nsteps > 0 && !(o.real_only && (comp & 1))) {
+                    const double2 *Ff = F + (size_t)f * four_dist + (size_t)im * nlat;
+                    const double2 fn = Ff[kl];
+                    const double2 fs = Ff[nlat - 1 - kl];
+                    const double w = lat_weight ? wqk[kl] : 1.0;
+                    if (comp == 0) v = (fn.x + fs.x) * w;
+                    else if (comp == 1) v = (fn.y + fs.y) * w;
+                    else if (comp == 2) v = (fn.x - fs.x) * w * xk[kl];
+                    else v = (fn.y - fs.y) * w * xk[kl];
+                }
+                Ereg[r][c][kg] = v;
             }
-            Ereg[r][kg] = v;
         }
     }
-    // block-uniform starting step
-    for (int off = 32; off >= 1; off >>= 1) my_ibeg = min(my_ibeg, __shfl_xor(my_ibeg, off));
+    for (int off = 32; off >= 1; off >>= 1) {
+        my_ibeg = min(my_ibeg, __shfl_xor(my_ibeg, off));
+        my_imax = max(my_imax, __shfl_xor(my_imax, off));
+    }
     if (lane == 0) s_ibeg[wave] = my_ibeg;
+    if (threadIdx.x < 2 * NCOL) (&carry[0][0])[threadIdx.x] = 0.0;
     __syncthreads();
     int ibeg = nsteps;
     for (int w = 0; w < NW; ++w) ibeg = min(ibeg, s_ibeg[w]);
@@ -392,107 +439,113 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
         zb[r] = 0.0;
         if (i0v[r] == ibeg) { za[r] = sdx[r]; zb[r] = sdy[r]; }
     }
-    if (threadIdx.x < 16) carry[threadIdx.x] = 0.0;
 
     const double qscale = (im == 0) ? 1.0 : mpos_scale;
-    // output: zero the steps below ibeg (functions negligible there)
-    // degrees written: o.mfun .. spec_ltop (zeros above ltr)
     const int nsteps_out = (spec_ltop >= o.mfun) ? (spec_ltop - o.mfun) / 2 + 1 : 0;
+    double2 *Qbase = Q + o.spec_base;
 
-    for (int i = 0; i < nsteps; i += 16) {
-        double4_t acc = (double4_t){0.0, 0.0, 0.0, 0.0};
-        if (i + 16 > ibeg) {
+    int b = 0;
+    for (int i = ibeg; i < nsteps; i += 16, ++b) {
+        double4_t acc[NCG];
 #pragma unroll
-            for (int r = 0; r < NLR; ++r) {
+        for (int c = 0; c < NCG; ++c) acc[c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int r = 0; r < NLR; ++r) {
+            if (i <= my_imax) {   // some lane of this wave starts its recurrence here
 #pragma unroll
                 for (int s = 0; s < 16; ++s) {
                     const int ii = i + s;
-                    if (ii == i0v[r] && ii != ibeg) { za[r] = sdx[r]; zb[r] = sdy[r]; }
-                    ytile[wave][s][lane] = (ii >= i0v[r] && ii < nsteps) ? zb[r] : 0.0;
-                    const double2 c = abm[min(ii, nsteps - 1)];
-                    const double zn = fma(fma(c.x, x2v[r], c.y), zb[r], za[r]);
+                    if (ii == i0v[r]) { za[r] = sdx[r]; zb[r] = sdy[r]; }
+                    ytile[wave][s][lane] = zb[r];
+                    const double2 cf = abm[min(ii, nsteps - 1)];
+                    const double zn = fma(fma(cf.x, x2v[r], cf.y), zb[r], za[r]);
                     za[r] = zb[r];
                     zb[r] = zn;
                 }
-                __builtin_amdgcn_s_waitcnt(0xc07f);
-                __builtin_amdgcn_wave_barrier();
+            } else {
 #pragma unroll
-                for (int kg = 0; kg < 16; ++kg) {
-                    const double a = ytile[wave][lane & 15][4 * kg + (lane >> 4)];
-                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ereg[r][kg], acc, 0, 0, 0);
+                for (int s = 0; s < 16; ++s) {
+                    ytile[wave][s][lane] = zb[r];
+                    const double2 cf = abm[min(i + s, nsteps - 1)];
+                    const double zn = fma(fma(cf.x, x2v[r], cf.y), zb[r], za[r]);
+                    za[r] = zb[r];
+                    zb[r] = zn;
                 }
-                __builtin_amdgcn_wave_barrier();
             }
-        }
-        // D layout (gfx950): acc[j] = D[step row 4*j + lane/16][col lane%16]
+            __builtin_amdgcn_s_waitcnt(0xc07f);
+            __builtin_amdgcn_wave_barrier();
 #pragma unroll
-        for (int j = 0; j < 4; ++j) red[wave][4 * j + (lane >> 4)][lane & 15] = acc[j];
-        __syncthreads();
-        // threads 0..255 of the block reduce 16x16 values; then post-mix
-        for (int e = threadIdx.x; e < 256; e += 64 * NW) {
-            const int row = e >> 4, c = e & 15;
-            double sum = 0.0;
+            for (int kg = 0; kg < 16; ++kg) {
+                const double a = ytile[wave][lane & 15][4 * kg + (lane >> 4)];
 #pragma unroll
-            for (int w = 0; w < NW; ++w) sum += red[w][row][c];
-            red[0][row][c] = sum;   // safe: each (row,c) read by exactly this thread above
+                for (int c = 0; c < NCG; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ereg[r][c][kg], acc[c], 0, 0, 0);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
+        // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
+        const int pb = b & 1;
+#pragma unroll
+        for (int c = 0; c < NCG; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[pb][wave][4 * j + (lane >> 4)][16 * c + (lane & 15)] = acc[c][j];
         __syncthreads();
-        // post-mix and store: thread handles (step row, field, re/im)
-        for (int e = threadIdx.x; e < 16 * 4 * 2; e += 64 * NW) {
-            const int row = e >> 3, f4 = (e >> 1) & 3, ri = e & 1;
-            const int f = fg * 4 + f4;
+        // sum over waves + post-mix; thread e handles (step row, column)
+        for (int e = threadIdx.x; e < 16 * NCOL; e += 64 * NW) {
+            const int row = e / NCOL, col = e - row * NCOL;
+            const int comp = col & 3;
+            const int f = fbase + (col >> 2);
+            double sum = 0.0, sprev = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) sum += red[pb][w][row][col];
+            if (row == 15 && comp < 2) carry[pb][col] = sum;
             const int ii = i + row;
-            if (f < nfields && ii < nsteps_out) {
-                const int le = o.mfun + 2 * ii;
-                double qe = 0.0, qo = 0.0;
-                if (ii < nsteps) {
-                    const double4 c = pmm[ii];
-                    const double ae = red[0][row][4 * f4 + ri];
-                    const double aprev = (row > 0) ? red[0][row - 1][4 * f4 + ri] : carry[4 * f4 + ri];
+            if (f >= nfields || ii >= nsteps_out) continue;
+            const int le = o.mfun + 2 * ii;
+            double v = 0.0;
+            if (ii < nsteps) {
+                const double4 cf = pmm[ii];
+                if (comp < 2) {
+                    if (row > 0) {
+#pragma unroll
+                        for (int w = 0; w < NW; ++w) sprev += red[pb][w][row - 1][col];
+                    } else {
+                        sprev = carry[pb ^ 1][col];
+                    }
                     const double cprev = (ii > 0) ? pmm[ii - 1].y : 0.0;
-                    qe = c.x * ae + cprev * aprev;
-                    qo = c.z * red[0][row][4 * f4 + 2 + ri];
-                }
-                // step nsteps (just past ltr) still receives the ue1 tail of step nsteps-1
-                double2 *Qf = Q + (size_t)f * spec_dist + o.spec_base;
-                if (le <= spec_ltop) {
-                    double v = (le <= ltr) ? qe * qscale : 0.0;
-                    if (o.real_only && ri) v = 0.0;
-                    reinterpret_cast<double *>(&Qf[le])[ri] = v;
-                }
-                if (le + 1 <= spec_ltop) {
-                    double v = (le + 1 <= ltr) ? qo * qscale : 0.0;
-                    if (o.real_only && ri) v = 0.0;
-                    reinterpret_cast<double *>(&Qf[le + 1])[ri] = v;
+                    v = cf.x * sum + cprev * sprev;
+                } else {
+                    v = cf.z * sum;
                 }
             }
+            const int ld = (comp < 2) ? le : le + 1;
+            if (ld <= spec_ltop) {
+                double out = (ld <= ltr) ? v * qscale : 0.0;
+                if (o.real_only && (comp & 1)) out = 0.0;
+                reinterpret_cast<double *>(&Qbase[(size_t)f * spec_dist + ld])[comp & 1] = out;
+            }
         }
-        __syncthreads();
-        if (threadIdx.x < 16) carry[threadIdx.x] = red[0][15][threadIdx.x];
-        __syncthreads();
     }
-    // orders/degrees with no step computed: zero-fill the remaining output degrees
-    for (int ii = ((nsteps + 15) & ~15) + threadIdx.x; ii < nsteps_out; ii += 64 * NW) {
+    // degrees not reached by the step loop: before ibeg (negligible functions) and after nsteps
+    for (int ii = threadIdx.x; ii < nsteps_out; ii += 64 * NW) {
+        if (ii >= ibeg && ii < ibeg + ((nsteps - ibeg + 15) & ~15)) continue;
         const int le = o.mfun + 2 * ii;
-        for (int f = fg * 4; f < min(nfields, fg * 4 + 4); ++f) {
-            double2 *Qf = Q + (size_t)f * spec_dist + o.spec_base;
+        for (int f = fbase; f < min(nfields, fbase + 4 * NCG); ++f) {
+            double2 *Qf = Qbase + (size_t)f * spec_dist;
             if (le <= spec_ltop) Qf[le] = make_double2(0.0, 0.0);
             if (le + 1 <= spec_ltop) Qf[le + 1] = make_double2(0.0, 0.0);
         }
     }
-    // the degree slot below mfun for vector m=0 (l=0) is zero
-    if (vector_mode && im == 0 && threadIdx.x == 0) {
-        for (int f = fg * 4; f < min(nfields, fg * 4 + 4); ++f) Q[(size_t)f * spec_dist + o.spec_base] = make_double2(0.0, 0.0);
+    if (vector_mode && im == 0 && threadIdx.x == 0) {   // degree 0 of the vector m=0 spectra
+        for (int f = fbase; f < min(nfields, fbase + 4 * NCG); ++f) Qbase[(size_t)f * spec_dist] = make_double2(0.0, 0.0);
     }
-    (void)m;
 }
 
-template <int NW, int NLR>
+template <int NW, int NLR, int NCG>
 static void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                           int spec_ltop, hipStream_t s) {
     const DeviceTables *t = a.t;
-    dim3 grid(a.mmax + 1, (a.nfields + 3) / 4);
-    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR>), grid, dim3(64 * NW), 0, s, t->x, t->x2, t->st, t->wq,
+    dim3 grid(a.mmax + 1, (a.nfields + 4 * NCG - 1) / (4 * NCG));
+    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG>), grid, dim3(64 * NW), 0, s, t->x, t->x2, t->st, t->wq,
                        t->step_ofs, t->ab, t->pm, t->i0, t->z0, F, four_dist, Q, spec_dist,
                        spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
                        a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale, a.lat_weight);
@@ -501,22 +554,30 @@ static void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dis
 void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                           int spec_ltop, hipStream_t s) {
     const int ntiles = (a.nlat_2 + 63) / 64;
-    // waves per block: a divisor of the tile count, at most 8
+    // waves per block: a divisor of the tile count, at most 8; tiles per wave at most 6
     int nw = 1;
     for (int d = 8; d >= 1; --d)
         if (ntiles % d == 0) { nw = d; break; }
     int nlr = ntiles / nw;
     if (nlr > 6) { nw = 8; nlr = (ntiles + 7) / 8; }
-#define ANAL_CASE(W, R) if (nw == W && nlr <= R) { launch_anal_t<W, R>(a, F, four_dist, Q, spec_dist, spec_ltop, s); return; }
-    ANAL_CASE(1, 1) ANAL_CASE(1, 2) ANAL_CASE(1, 3) ANAL_CASE(1, 5) ANAL_CASE(1, 7)
-    ANAL_CASE(2, 1) ANAL_CASE(2, 3) ANAL_CASE(2, 5)
-    ANAL_CASE(3, 1) ANAL_CASE(3, 3) ANAL_CASE(3, 5)
-    ANAL_CASE(4, 1) ANAL_CASE(4, 2) ANAL_CASE(4, 3) ANAL_CASE(4, 5)
-    ANAL_CASE(5, 1) ANAL_CASE(5, 3)
-    ANAL_CASE(6, 1) ANAL_CASE(6, 2) ANAL_CASE(6, 3) ANAL_CASE(6, 5)
-    ANAL_CASE(7, 1) ANAL_CASE(7, 3)
-    ANAL_CASE(8, 1) ANAL_CASE(8, 2) ANAL_CASE(8, 3) ANAL_CASE(8, 4) ANAL_CASE(8, 6) ANAL_CASE(8, 8)
-    ANAL_CASE(8, 12) ANAL_CASE(8, 16)
+    static const int ncg_env = getenv("SHTNS_AMD_ANAL_NCG") ? atoi(getenv("SHTNS_AMD_ANAL_NCG")) : 0;
+    bool wide = (a.nfields > 4) && (nlr <= 2);   // 8 fields per block when registers allow
+    if (ncg_env == 1) wide = false;
+    if (ncg_env == 2 && nlr <= 2) wide = (a.nfields > 4);
+#define ANAL_CASE(W, R)                                                                              \
+    if (nw == W && nlr <= R) {                                                                       \
+        if (wide) launch_anal_t<W, R, 2>(a, F, four_dist, Q, spec_dist, spec_ltop, s);              \
+        else launch_anal_t<W, R, 1>(a, F, four_dist, Q, spec_dist, spec_ltop, s);                   \
+        return;                                                                                      \
+    }
+    ANAL_CASE(1, 1) ANAL_CASE(1, 2) ANAL_CASE(1, 3) ANAL_CASE(1, 6)
+    ANAL_CASE(2, 1) ANAL_CASE(2, 3) ANAL_CASE(2, 6)
+    ANAL_CASE(3, 1) ANAL_CASE(3, 2) ANAL_CASE(3, 4) ANAL_CASE(3, 6)
+    ANAL_CASE(4, 1) ANAL_CASE(4, 2) ANAL_CASE(4, 3) ANAL_CASE(4, 6)
+    ANAL_CASE(5, 1) ANAL_CASE(5, 3) ANAL_CASE(5, 6)
+    ANAL_CASE(6, 1) ANAL_CASE(6, 2) ANAL_CASE(6, 3) ANAL_CASE(6, 6)
+    ANAL_CASE(7, 1) ANAL_CASE(7, 3) ANAL_CASE(7, 6)
+    ANAL_CASE(8, 1) ANAL_CASE(8, 2) ANAL_CASE(8, 3) ANAL_CASE(8, 4) ANAL_CASE(8, 6)
 #undef ANAL_CASE
     fprintf(stderr, "shtns_amd: no analysis kernel for nlat_2=%d\n", a.nlat_2);
 }
@@ -585,11 +646,19 @@ __device__ inline void dif_butterfly(double2 *a, int q, int n1, int twm, const d
     a[0] = y[0];
 #pragma unroll
     for (int k = 1; k < P; ++k) {
-        const int e = (int)(((long)n1 * k * twm) % N);
+        const int e = n1 * k * twm;   // < N since n1 < Ns/P, k < P, twm = N/Ns
         double2 w = roots[e];
         w.y *= sgn;
         a[k * q] = cmul(y[k], w);
     }
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous range of latitude
+// groups so concurrently running blocks on one XCD write neighbouring latitudes (whose
+// theta-contiguous rows share cache lines in that XCD's L2).
+__device__ inline int xcd_group(int bid, int G) {
+    if (G % 8) return bid;
+    return (bid % 8) * (G / 8) + bid / 8;
 }
 
 __device__ void fft_dif_inplace(double2 *buf, int T, int M, const FftPlan &plan, const double2 *__restrict__ roots,
@@ -626,7 +695,7 @@ __global__ __launch_bounds__(256) void fft_synth_kernel(const double2 *__restric
     extern __shared__ double2 buf[];
     const int M = plan.M;
     const int N = 2 * M;
-    const int lat0 = blockIdx.x * T;
+    const int lat0 = xcd_group(blockIdx.x, gridDim.x) * T;
     const int f = blockIdx.y;
     const double2 *Ff = F + (size_t)f * four_dist;
     double *Vf = V + (size_t)f * spat_dist;
@@ -668,7 +737,7 @@ __global__ __launch_bounds__(256) void fft_anal_kernel(const double *__restrict_
                                                        int T) {
     extern __shared__ double2 buf[];
     const int M = plan.M;
-    const int lat0 = blockIdx.x * T;
+    const int lat0 = xcd_group(blockIdx.x, gridDim.x) * T;
     const int f = blockIdx.y;
     const double *Vf = V + (size_t)f * spat_dist;
     double2 *Ff = F + (size_t)f * four_dist;
@@ -705,10 +774,12 @@ __global__ __launch_bounds__(256) void fft_anal_kernel(const double *__restrict_
 }
 
 static int fft_lat_per_block(int M) {
-    // LDS budget of 64 KB per block
-    int T = (int)(65536 / (16L * M));
+    static const int t_env = getenv("SHTNS_AMD_FFT_T") ? atoi(getenv("SHTNS_AMD_FFT_T")) : 0;
+    int T = (int)(65536 / (16L * M));   // LDS budget of 64 KB per block by default
+    if (t_env > 0) T = t_env;
+    while (T > 1 && 16L * M * T > 160 * 1024) --T;
     if (T < 1) T = 1;
-    if (T > 4) T = 4;
+    if (T > 8) T = 8;
     return T;
 }
 
