@@ -70,7 +70,7 @@ struct shtns_info {
 
 /* index macros (shtns.h:104-106, 120) */
 #define LiM(shtns, l, im) ((((im) * (2 * shtns->lmax + 2 - ((im) + 1) * shtns->mres)) >> 1) + (l))
-#define LM(shtns, l, m) (((((unsigned short)(m)) / shtns->mres) * (2 * shtns->lmax + 2 - ((m) + shtns->mres))) >> 1) + (l))
+#define LM(shtns, l, m) ((((((unsigned short)(m)) / shtns->mres) * (2 * shtns->lmax + 2 - ((m) + shtns->mres))) >> 1) + (l))
 #define NSPAT_ALLOC(shtns) (shtns->nspat)
 #define PHI_DEG(shtns, ip) (360. / (shtns->nphi * shtns->mres)) * (ip)
 #define PHI_RAD(shtns, ip) (2. * M_PI / (shtns->nphi * shtns->mres)) * (ip)

@@ -1,0 +1,28 @@
+/* A C99 caller written the way SHTns users write them (shtns.h only, public struct
+ * fields and macros), built with gcc against libshtns_amd.so.  Runs host-side calls only,
+ * so it works without a GPU; prints key=value lines checked by tests/test_abi.py. */
+#include <math.h>
+#include <stddef.h>
+#include <stdio.h>
+
+#include "shtns.h"
+
+int main(void) {
+    shtns_cfg c = shtns_create(31, 31, 1, sht_orthonormal);
+    int nspat = shtns_set_grid(c, sht_gauss | SHT_THETA_CONTIGUOUS, 0.0, 32, 64);
+    double w[16];
+    int nw = shtns_gauss_wts(c, w);
+    double yl[32];
+    legendre_sphPlm_array(c, 31, 2, 0.3, yl);
+    printf("nlm=%u lmax=%u mmax=%u mres=%u nlat=%u nlat_2=%u nphi=%u nspat=%u ret=%d\n", c->nlm, c->lmax, c->mmax,
+           c->mres, c->nlat, c->nlat_2, c->nphi, c->nspat, nspat);
+    printf("LM11=%d LiM=%d nlm_calc=%ld nlm_cplx=%u\n", (int)LM(c, 1, 1), (int)LiM(c, 5, 3), nlm_calc(31, 31, 1), c->nlm_cplx);
+    printf("ct0=%.17g st0=%.17g li5=%u mi40=%u nw=%d w0=%.17g y2=%.17g\n", c->ct[0], c->st[0], c->li[5], c->mi[40], nw,
+           w[0], yl[0]);
+    printf("sh00=%.17g sh10=%.17g sh11=%.17g\n", sh00_1(c), sh10_ct(c), sh11_st(c));
+    printf("off_nlat=%zu off_li=%zu off_ct=%zu off_nlm_cplx=%zu\n", offsetof(struct shtns_info, nlat),
+           offsetof(struct shtns_info, li), offsetof(struct shtns_info, ct), offsetof(struct shtns_info, nlm_cplx));
+    shtns_destroy(c);
+    shtns_reset();
+    return 0;
+}
