@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--layout", choices=["theta", "phi"], default="theta",
+                    help="spatial layout (the metric is quoted on theta-contiguous)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -90,10 +92,11 @@ def main():
     lib = _lib.load()
     dev = torch.device("cuda:0")
     s = shtns_amd.sht(LMAX)
-    s.set_grid(NLAT, NPHI, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+    lay = shtns_amd.SHT_THETA_CONTIGUOUS if args.layout == "theta" else shtns_amd.SHT_PHI_CONTIGUOUS
+    s.set_grid(NLAT, NPHI, flags=lay | shtns_amd.sht_gauss, polar_opt=0.0)
     nlm, nspat = s.nlm, s.nspat
     q = torch.from_numpy(spectral_batch(LMAX, LMAX, BATCH, seed=SEED)).to(dev)
-    v = torch.empty((BATCH, NPHI, NLAT), dtype=torch.float64, device=dev)
+    v = torch.empty((BATCH,) + tuple(s.spat_shape), dtype=torch.float64, device=dev)
     q2 = torch.empty_like(q)
     stream = torch.cuda.current_stream(dev)
     lib.cushtns_set_streams(s._cfg, ctypes.c_void_p(stream.cuda_stream), ctypes.c_void_p(stream.cuda_stream))
@@ -154,7 +157,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (SplitMix64/Box-Muller a_lm ~ N(0,(l+1)^-2), seed 1+field)",
         "config": {"workload": "SH_to_spat+spat_to_SH, lmax=mmax=1023, Gauss nlat=1536 nphi=3072, "
-                               "theta-contiguous, batch 16 FP64 fields",
+                               f"{args.layout}-contiguous, batch 16 FP64 fields",
                    "lmax": LMAX, "nlat": NLAT, "nphi": NPHI, "batch": BATCH, "parallelism": "single GPU"},
         "fp64_frac_of_peak_whole_pair": round(alg_total / elapsed / 1e12 / FP64_PEAK_TFLOPS, 4),
         "round_trip_rel_l2": err,

@@ -734,14 +734,19 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
-    double2 *F = (double2 *)scratch(c, (size_t)nfields * four_dist * sizeof(double2));
-    if (!F) return;
+    const size_t fbytes = (size_t)nfields * four_dist * sizeof(double2);
+    const size_t qbytes = synth_qmix_doubles(nfields, c->nsteps_total) * sizeof(double);
+    char *base = (char *)scratch(c, fbytes + qbytes);
+    if (!base) return;
+    double2 *F = (double2 *)base;
+    double *qmix = (double *)(base + fbytes);
     LegendreArgs a = leg_args(c, ltr, imlim, nfields);
     a.mpos_scale = mpos_scale;
     a.lat_weight = lat_weight;
     {
         ProfScope ps(c, PK_LEG_SYNTH);
-        launch_legendre_synth(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, F, four_dist, c->comp_stream);
+        launch_legendre_synth(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, F, four_dist, qmix,
+                              c->nsteps_total, c->comp_stream);
     }
     long sp, st;
     spat_strides(c, sp, st);
@@ -787,15 +792,17 @@ static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, d
     const long vw_dist = nlm_calc(c->lmax + 1, c->mmax, c->mres) + 2;
     const size_t fbytes = (size_t)2 * nfields * four_dist * sizeof(double2);
     const size_t vbytes = (size_t)2 * nfields * vw_dist * sizeof(double2);
-    char *base = (char *)scratch(c, fbytes + vbytes);
+    const size_t qbytes = synth_qmix_doubles(2 * nfields, c->nsteps_total) * sizeof(double);
+    char *base = (char *)scratch(c, fbytes + vbytes + qbytes);
     if (!base) return;
     double2 *F = (double2 *)base;
     double2 *VW = (double2 *)(base + fbytes);
+    double *qmix = (double *)(base + fbytes + vbytes);
     launch_sphtor_to_vw(reinterpret_cast<const double2 *>(S), reinterpret_cast<const double2 *>(T), spec_dist, VW,
                         vw_dist, c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);
     LegendreArgs a = leg_args(c, ltr + 1, imlim, 2 * nfields);
     a.vector_mode = 1;
-    launch_legendre_synth(a, VW, vw_dist, c->lmax + 1, F, four_dist, c->comp_stream);
+    launch_legendre_synth(a, VW, vw_dist, c->lmax + 1, F, four_dist, qmix, c->nsteps_total, c->comp_stream);
     long sp, st;
     spat_strides(c, sp, st);
     launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, 2 * four_dist, Vt, spat_dist, sp, st,

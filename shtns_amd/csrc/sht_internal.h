@@ -97,8 +97,10 @@ struct LegendreArgs {
 };
 
 // Fourier buffer layout: F[field][im][nlat] complex (north to south).
+// qmix: scratch of synth_qmix_doubles(nfields, nsteps_total) doubles (pre-mixed coefficients)
 void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
-                           double2 *F, long four_dist, hipStream_t s);
+                           double2 *F, long four_dist, double *qmix, int nsteps_total, hipStream_t s);
+size_t synth_qmix_doubles(int nfields, int nsteps_total);
 void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dist,
                           double2 *Qlm, long spec_dist, int spec_ltop, hipStream_t s);
 // Spatial element (ip, it) of a field sits at ip*sp + it*st (theta-contiguous: sp=nlat, st=1).

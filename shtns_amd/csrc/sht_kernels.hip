@@ -112,6 +112,13 @@ void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_
 // ---------------------------------------------------------------------------------------
 // Helpers shared by the Legendre kernels
 // ---------------------------------------------------------------------------------------
+// Read-only table viewed through the constant address space: loads with wave-uniform
+// indices become scalar (SMEM) loads and stay off the vector-memory counters.
+struct UniformTable2 {
+    const __attribute__((address_space(4))) double *p;
+    __device__ explicit UniformTable2(const double2 *q) : p((const __attribute__((address_space(4))) double *)q) {}
+    __device__ double2 operator[](int i) const { return make_double2(p[2 * i], p[2 * i + 1]); }
+};
 struct OrderInfo {
     int slot;       // table slot (im, or mmax+1 for the m=1 functions of vector m=0)
     int mfun;       // order of the generated functions
@@ -140,40 +147,89 @@ __device__ inline OrderInfo order_info(int im, int lmax_spec, int mmax, int mres
 }
 
 // ---------------------------------------------------------------------------------------
-// Synthesis: F[f][im][k] = sum_l Q_lm P_lm(x_k)   (both hemispheres)
-// Block = 4 waves for one order im, 4*NCG fields (blockIdx.y) and 4 latitude tiles of 64
-// pairs (blockIdx.z); wave w owns tile blockIdx.z*4 + w, lane = latitude pair.
-//   * the block pre-mixes the coefficients of 32 steps at a time into LDS in the layout of
-//     the MFMA B operand: qs[step][col], col = 16*cg + 4*field + (E.re, E.im, O.re, O.im),
-//     E_i = ue_i Q_{m+2i} + ue1_i Q_{m+2i+2},  O_i = uo_i Q_{m+2i+1};
-//   * per 4 steps each lane advances its recurrence (z_i for its latitude) into a wave-private
-//     LDS tile; the tile is read back as the A operand (16 latitudes x 4 steps) and
-//     D[16 lat][16 col] += A * B runs for 4 row tiles x NCG column groups on the matrix cores.
-//   * the accumulators go straight to F: lanes holding E and O of one field are lane^2
-//     partners (north = E + x O, south = E - x O).
+// Synthesis, step 1 (pre-mix): the coefficients are rearranged once per transform into the
+// layout of the MFMA B operand, so the Legendre kernel reads them with plain coalesced
+// loads and needs no block barrier:
+//   qmix[g][slot steps][NCOL],  col = 16*cg + 4*field + (E.re, E.im, O.re, O.im),
+//   E_i = ue_i Q_{m+2i} + ue1_i Q_{m+2i+2},  O_i = uo_i Q_{m+2i+1}    (g = group of 4*NCG fields)
+// ---------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void premix_kernel(const int *__restrict__ step_ofs, const double4 *__restrict__ pm,
+                                                     const double2 *__restrict__ Q, long spec_dist, int lmax_spec,
+                                                     double *__restrict__ qmix, int nsteps_total, int ncol, int mmax,
+                                                     int mres, int ltr, int imlim, int nfields, int vector_mode,
+                                                     double mpos_scale) {
+    const int im = blockIdx.x;
+    const int g = blockIdx.y;
+    if (im > imlim) return;
+    const OrderInfo o = order_info(im, lmax_spec, mmax, mres, ltr, vector_mode);
+    const int nsteps = o.nsteps;
+    const double4 *pmm = pm + step_ofs[o.slot];
+    double *dst = qmix + ((size_t)g * nsteps_total + step_ofs[o.slot]) * ncol;
+    const double qscale = (im == 0) ? 1.0 : mpos_scale;
+    const int fpg = ncol / 4;   // fields per group
+    // rows nsteps .. end of the slot are written as zeros: the Legendre kernel reads them
+    // unconditionally (clamped to the slot) when it runs past the last step
+    const int slot_rows = step_ofs[o.slot + 1] - step_ofs[o.slot];
+    for (int e = blockIdx.z * blockDim.x + threadIdx.x; e < slot_rows * ncol; e += gridDim.z * blockDim.x) {
+        const int ii = e / ncol, col = e - ii * ncol;
+        const int f = g * fpg + (col >> 2);
+        const int comp = col & 3;
+        double v = 0.0;
+        if (ii < nsteps && f < nfields && !(o.real_only && (comp & 1))) {
+            const double2 *Qf = Q + (size_t)f * spec_dist + o.spec_base;
+            const int le = o.mfun + 2 * ii;
+            const double4 cf = pmm[ii];
+            if (comp < 2) {
+                const double2 q0 = Qf[le];
+                double a = (comp == 0) ? q0.x : q0.y, b = 0.0;
+                if (le + 2 <= ltr) {
+                    const double2 q2 = Qf[le + 2];
+                    b = (comp == 0) ? q2.x : q2.y;
+                }
+                v = cf.x * a + cf.y * b;
+            } else if (le + 1 <= ltr) {
+                const double2 q1 = Qf[le + 1];
+                v = cf.z * ((comp == 2) ? q1.x : q1.y);
+            }
+            v *= qscale;
+        }
+        dst[e] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Synthesis, step 2: F[f][im][k] = sum_l Q_lm P_lm(x_k)   (both hemispheres)
+// One wave = one order im, 4*NCG fields, one latitude tile of 64 pairs (lane = pair); waves
+// are independent (no block barrier).  Software pipeline per 4-step block:
+//   A operands (z of 16 latitudes x 4 steps) read back from the wave's LDS tile,
+//   4 x NCG MFMAs D[16 lat][16 col] += A * B,
+//   then, while the matrix cores work, the recurrence of the next 4 steps is advanced on
+//   the vector ALUs into the other LDS tile and the next B operands are loaded.
+// The accumulators go straight to F: lanes holding E and O of one field are lane^2
+// partners (north = E + x O, south = E - x O).
 // ---------------------------------------------------------------------------------------
 constexpr int kSynthWaves = 4;
-constexpr int kSynthChunk = 32;   // steps pre-mixed per LDS fill
 
 template <int NCG>
 __global__ __launch_bounds__(256) void legendre_synth_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
-    const double4 *__restrict__ pm, const int *__restrict__ i0tab, const double2 *__restrict__ z0tab,
-    const double2 *__restrict__ Q, long spec_dist, int lmax_spec, double2 *__restrict__ F, long four_dist,
-    int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
-    double mpos_scale, int lat_weight) {
+    const int *__restrict__ i0tab, const double2 *__restrict__ z0tab, const double *__restrict__ qmix,
+    int nsteps_total, int lmax_spec, double2 *__restrict__ F, long four_dist, int mmax, int mres, int nlat,
+    int nlat_2, int ltr, int imlim, int nfields, int vector_mode, int lat_weight, int ntiles) {
     constexpr int NCOL = 16 * NCG;
-    __shared__ double qs[kSynthChunk][NCOL + 1];
-    __shared__ double ytile[kSynthWaves][4][64];
-    __shared__ int s_red[kSynthWaves];
+    __shared__ double ytile[kSynthWaves][2][4][64];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int im = blockIdx.x;
-    const int fbase = blockIdx.y * 4 * NCG;
-    const int tile = blockIdx.z * kSynthWaves + wave;
-    const int k = tile * 64 + lane;   // latitude pair owned by this lane
+    // heavy orders first: block x -> (order, tile group); all tile groups of an order adjacent
+    const int ntg = (ntiles + kSynthWaves - 1) / kSynthWaves;
+    const int im = blockIdx.x / ntg;
+    const int tile = (blockIdx.x - im * ntg) * kSynthWaves + wave;
+    const int g = blockIdx.y;
+    const int fbase = g * 4 * NCG;
+    if (tile >= ntiles) return;   // no block-level synchronization in this kernel
+    const int k = tile * 64 + lane;
     const bool kvalid = (k < nlat_2);
     const int kk = kvalid ? k : (nlat_2 - 1);
 
@@ -187,8 +243,9 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     const int nsteps = (im <= imlim) ? o.nsteps : 0;
     if (nsteps > 0) {
         const int sofs = step_ofs[o.slot];
-        const double2 *abm = ab + sofs;
-        const double4 *pmm = pm + sofs;
+        const int slot_rows = step_ofs[o.slot + 1] - sofs;   // > nsteps: the last rows of qmix are zero
+        const UniformTable2 abm(ab + sofs);   // uniform indices -> scalar loads
+        const double *qb = qmix + ((size_t)g * nsteps_total + sofs) * NCOL;
         const double x2 = x2k[kk];
         int i0 = kvalid ? i0tab[(size_t)o.slot * nlat_2 + kk] : nsteps;
         double2 sd = z0tab[(size_t)o.slot * nlat_2 + kk];
@@ -198,88 +255,67 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
             sd.y *= inv;
         }
         if (i0 > nsteps) i0 = nsteps;
-        // block-uniform first step and wave-uniform last seed step
         int wmin = i0, wmax = i0;
         for (int off = 32; off >= 1; off >>= 1) {
             wmin = min(wmin, __shfl_xor(wmin, off));
             wmax = max(wmax, __shfl_xor(wmax, off));
         }
-        if (lane == 0) s_red[wave] = wmin;
-        __syncthreads();
-        int ibeg = nsteps;
-        for (int w = 0; w < kSynthWaves; ++w) ibeg = min(ibeg, s_red[w]);
-        ibeg &= ~3;
+        // wave-uniform: keeps loop control and coefficient loads on the scalar unit
+        wmin = __builtin_amdgcn_readfirstlane(wmin);
+        wmax = __builtin_amdgcn_readfirstlane(wmax);
+        const int ibeg = wmin & ~3;
         double za = 0.0, zb = 0.0;
         if (i0 == ibeg) { za = sd.x; zb = sd.y; }
-        const double qscale = (im == 0) ? 1.0 : mpos_scale;
 
-        for (int c0 = ibeg; c0 < nsteps; c0 += kSynthChunk) {
-            __syncthreads();   // previous chunk's B operands consumed
-            // pre-mix steps c0 .. c0+31 into qs
-            for (int e = threadIdx.x; e < kSynthChunk * NCOL; e += 256) {
-                const int sl = e / NCOL, col = e - sl * NCOL;
-                const int ii = c0 + sl;
-                const int f = fbase + (col >> 2);
-                const int comp = col & 3;
-                double v = 0.0;
-                if (ii < nsteps && f < nfields && !(o.real_only && (comp & 1))) {
-                    const double2 *Qf = Q + (size_t)f * spec_dist + o.spec_base;
-                    const int le = o.mfun + 2 * ii;
-                    const double4 cf = pmm[ii];
-                    if (comp < 2) {
-                        const double2 q0 = Qf[le];
-                        double a = (comp == 0) ? q0.x : q0.y, b = 0.0;
-                        if (le + 2 <= ltr) {
-                            const double2 q2 = Qf[le + 2];
-                            b = (comp == 0) ? q2.x : q2.y;
-                        }
-                        v = cf.x * a + cf.y * b;
-                    } else if (le + 1 <= ltr) {
-                        const double2 q1 = Qf[le + 1];
-                        v = cf.z * ((comp == 2) ? q1.x : q1.y);
-                    }
-                    v *= qscale;
+        // advance 4 recurrence steps starting at step i, writing z_i..z_{i+3} to tile buffer
+        auto advance = [&](int i, int buf) {
+            double2 cf[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) cf[s] = abm[min(i + s, nsteps - 1)];
+            if (i <= wmax) {   // some lane of this wave starts in this block
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (i + s == i0) { za = sd.x; zb = sd.y; }
+                    ytile[wave][buf][s][lane] = zb;
+                    const double zn = fma(fma(cf[s].x, x2, cf[s].y), zb, za);
+                    za = zb;
+                    zb = zn;
                 }
-                qs[sl][col] = v;
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    ytile[wave][buf][s][lane] = zb;
+                    const double zn = fma(fma(cf[s].x, x2, cf[s].y), zb, za);
+                    za = zb;
+                    zb = zn;
+                }
             }
-            __syncthreads();
-            const int cend = min(c0 + kSynthChunk, nsteps);
-            for (int i = c0; i < cend; i += 4) {
-                if (i <= wmax) {   // some lane of this wave starts in this 4-step block
+        };
+        auto load_b = [&](int i, double *b) {
+            const int ii = min(i + (lane >> 4), slot_rows - 1);   // rows >= nsteps hold zeros
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        const int ii = i + s;
-                        if (ii == i0) { za = sd.x; zb = sd.y; }
-                        ytile[wave][s][lane] = zb;
-                        const double2 cf = abm[min(ii, nsteps - 1)];
-                        const double zn = fma(fma(cf.x, x2, cf.y), zb, za);
-                        za = zb;
-                        zb = zn;
-                    }
-                } else {
+            for (int c = 0; c < NCG; ++c) b[c] = qb[(size_t)ii * NCOL + c * 16 + (lane & 15)];
+        };
+
+        double bcur[NCG], bnxt[NCG];
+        advance(ibeg, 0);
+        load_b(ibeg, bcur);
+        int buf = 0;
+        for (int i = ibeg; i < nsteps; i += 4) {
+            load_b(i + 4, bnxt);
+            __builtin_amdgcn_s_waitcnt(0xc07f);   // LDS tile of this block written
+            __builtin_amdgcn_wave_barrier();
+            double a[4];
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) {
-                        ytile[wave][s][lane] = zb;
-                        const double2 cf = abm[min(i + s, nsteps - 1)];
-                        const double zn = fma(fma(cf.x, x2, cf.y), zb, za);
-                        za = zb;
-                        zb = zn;
-                    }
-                }
-                __builtin_amdgcn_s_waitcnt(0xc07f);
-                __builtin_amdgcn_wave_barrier();
-                double a[4];
+            for (int r = 0; r < 4; ++r) a[r] = ytile[wave][buf][lane >> 4][r * 16 + (lane & 15)];
 #pragma unroll
-                for (int r = 0; r < 4; ++r) a[r] = ytile[wave][lane >> 4][r * 16 + (lane & 15)];
-                const int sl = i - c0 + (lane >> 4);
+            for (int c = 0; c < NCG; ++c)
 #pragma unroll
-                for (int c = 0; c < NCG; ++c) {
-                    const double b = qs[sl][c * 16 + (lane & 15)];
+                for (int r = 0; r < 4; ++r) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], bcur[c], acc[r][c], 0, 0, 0);
+            if (i + 4 < nsteps) advance(i + 4, buf ^ 1);
+            buf ^= 1;
 #pragma unroll
-                    for (int r = 0; r < 4; ++r) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[r], b, acc[r][c], 0, 0, 0);
-                }
-                __builtin_amdgcn_wave_barrier();
-            }
+            for (int c = 0; c < NCG; ++c) bcur[c] = bnxt[c];
         }
     }
 
@@ -313,30 +349,36 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     }
 }
 
+int synth_ncg(int nfields) { return nfields > 8 ? 4 : (nfields > 4 ? 2 : 1); }
+
+size_t synth_qmix_doubles(int nfields, int nsteps_total) {
+    const int ncg = synth_ncg(nfields);
+    const int ng = (nfields + 4 * ncg - 1) / (4 * ncg);
+    return (size_t)ng * nsteps_total * 16 * ncg;
+}
+
 void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
-                           double2 *F, long four_dist, hipStream_t s) {
+                           double2 *F, long four_dist, double *qmix, int nsteps_total, hipStream_t s) {
     const DeviceTables *t = a.t;
     const int ntiles = (a.nlat_2 + 63) / 64;
-    const int nz = (ntiles + kSynthWaves - 1) / kSynthWaves;
-    if (a.nfields > 8) {
-        dim3 grid(a.mmax + 1, (a.nfields + 15) / 16, nz);
-        hipLaunchKernelGGL((legendre_synth_kernel<4>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq,
-                           t->step_ofs, t->ab, t->pm, t->i0, t->z0, Qlm, spec_dist, spec_ltop, F, four_dist, a.mmax,
-                           a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale,
-                           a.lat_weight);
-    } else if (a.nfields > 4) {
-        dim3 grid(a.mmax + 1, 1, nz);
-        hipLaunchKernelGGL((legendre_synth_kernel<2>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq,
-                           t->step_ofs, t->ab, t->pm, t->i0, t->z0, Qlm, spec_dist, spec_ltop, F, four_dist, a.mmax,
-                           a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale,
-                           a.lat_weight);
-    } else {
-        dim3 grid(a.mmax + 1, 1, nz);
-        hipLaunchKernelGGL((legendre_synth_kernel<1>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq,
-                           t->step_ofs, t->ab, t->pm, t->i0, t->z0, Qlm, spec_dist, spec_ltop, F, four_dist, a.mmax,
-                           a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale,
-                           a.lat_weight);
+    const int ntg = (ntiles + kSynthWaves - 1) / kSynthWaves;
+    const int ncg = synth_ncg(a.nfields);
+    const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
+    {
+        dim3 grid(a.mmax + 1, ng, 4);
+        hipLaunchKernelGGL(premix_kernel, grid, dim3(256), 0, s, t->step_ofs, t->pm, Qlm, spec_dist, spec_ltop, qmix,
+                           nsteps_total, 16 * ncg, a.mmax, a.mres, a.ltr, a.imlim, a.nfields, a.vector_mode,
+                           a.mpos_scale);
     }
+    dim3 grid((a.mmax + 1) * ntg, ng);
+#define SYNTH_LAUNCH(NC)                                                                                         \
+    hipLaunchKernelGGL((legendre_synth_kernel<NC>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq, \
+                       t->step_ofs, t->ab, t->i0, t->z0, qmix, nsteps_total, spec_ltop, F, four_dist, a.mmax,    \
+                       a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.lat_weight, ntiles)
+    if (ncg == 4) SYNTH_LAUNCH(4);
+    else if (ncg == 2) SYNTH_LAUNCH(2);
+    else SYNTH_LAUNCH(1);
+#undef SYNTH_LAUNCH
 }
 
 // ---------------------------------------------------------------------------------------
@@ -373,7 +415,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, vector_mode);
     const int nsteps = (im <= imlim) ? o.nsteps : 0;
     const int sofs = step_ofs[o.slot];
-    const double2 *abm = ab + sofs;
+    const UniformTable2 abm(ab + sofs);   // uniform indices -> scalar loads
     const double4 *pmm = pm + sofs;
 
     double za[NLR], zb[NLR], x2v[NLR], sdx[NLR], sdy[NLR];
@@ -427,12 +469,13 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
         my_ibeg = min(my_ibeg, __shfl_xor(my_ibeg, off));
         my_imax = max(my_imax, __shfl_xor(my_imax, off));
     }
+    my_imax = __builtin_amdgcn_readfirstlane(my_imax);
     if (lane == 0) s_ibeg[wave] = my_ibeg;
     if (threadIdx.x < 2 * NCOL) (&carry[0][0])[threadIdx.x] = 0.0;
     __syncthreads();
     int ibeg = nsteps;
     for (int w = 0; w < NW; ++w) ibeg = min(ibeg, s_ibeg[w]);
-    ibeg &= ~15;
+    ibeg = __builtin_amdgcn_readfirstlane(ibeg) & ~15;   // block-uniform, scalar from here on
 #pragma unroll
     for (int r = 0; r < NLR; ++r) {
         za[r] = 0.0;
@@ -446,6 +489,9 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
 
     int b = 0;
     for (int i = ibeg; i < nsteps; i += 16, ++b) {
+        double2 cf[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
         double4_t acc[NCG];
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] = (double4_t){0.0, 0.0, 0.0, 0.0};
@@ -457,8 +503,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
                     const int ii = i + s;
                     if (ii == i0v[r]) { za[r] = sdx[r]; zb[r] = sdy[r]; }
                     ytile[wave][s][lane] = zb[r];
-                    const double2 cf = abm[min(ii, nsteps - 1)];
-                    const double zn = fma(fma(cf.x, x2v[r], cf.y), zb[r], za[r]);
+                    const double zn = fma(fma(cf[s].x, x2v[r], cf[s].y), zb[r], za[r]);
                     za[r] = zb[r];
                     zb[r] = zn;
                 }
@@ -466,8 +511,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
 #pragma unroll
                 for (int s = 0; s < 16; ++s) {
                     ytile[wave][s][lane] = zb[r];
-                    const double2 cf = abm[min(i + s, nsteps - 1)];
-                    const double zn = fma(fma(cf.x, x2v[r], cf.y), zb[r], za[r]);
+                    const double zn = fma(fma(cf[s].x, x2v[r], cf[s].y), zb[r], za[r]);
                     za[r] = zb[r];
                     zb[r] = zn;
                 }
@@ -558,12 +602,14 @@ void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dis
     int nw = 1;
     for (int d = 8; d >= 1; --d)
         if (ntiles % d == 0) { nw = d; break; }
+    static const int nw_env = getenv("SHTNS_AMD_ANAL_NW") ? atoi(getenv("SHTNS_AMD_ANAL_NW")) : 0;
+    if (nw_env > 0 && nw_env <= 8 && ntiles % nw_env == 0) nw = nw_env;
     int nlr = ntiles / nw;
     if (nlr > 6) { nw = 8; nlr = (ntiles + 7) / 8; }
     static const int ncg_env = getenv("SHTNS_AMD_ANAL_NCG") ? atoi(getenv("SHTNS_AMD_ANAL_NCG")) : 0;
-    bool wide = (a.nfields > 4) && (nlr <= 2);   // 8 fields per block when registers allow
+    bool wide = (a.nfields > 4) && (nlr <= 3);   // 8 fields per block when registers allow
     if (ncg_env == 1) wide = false;
-    if (ncg_env == 2 && nlr <= 2) wide = (a.nfields > 4);
+    if (ncg_env == 2 && nlr <= 3) wide = (a.nfields > 4);
 #define ANAL_CASE(W, R)                                                                              \
     if (nw == W && nlr <= R) {                                                                       \
         if (wide) launch_anal_t<W, R, 2>(a, F, four_dist, Q, spec_dist, spec_ltop, s);              \
@@ -688,13 +734,58 @@ __device__ void fft_dif_inplace(double2 *buf, int T, int M, const FftPlan &plan,
     }
 }
 
+// Compile-time radix plan (same factor order as the host's factor_fft: 4s, then 2, 3, 5, 7),
+// so kernels specialized on M have all butterfly index arithmetic on constants.
+struct CPlan {
+    int n;
+    int r[24];
+};
+__host__ __device__ constexpr CPlan make_cplan(int M) {
+    CPlan p{0, {}};
+    int n = M;
+    while (n % 4 == 0) { p.r[p.n++] = 4; n /= 4; }
+    while (n % 2 == 0) { p.r[p.n++] = 2; n /= 2; }
+    while (n % 3 == 0) { p.r[p.n++] = 3; n /= 3; }
+    while (n % 5 == 0) { p.r[p.n++] = 5; n /= 5; }
+    while (n % 7 == 0) { p.r[p.n++] = 7; n /= 7; }
+    return p;
+}
+__host__ __device__ constexpr int cplan_ns(int M, int S) {
+    CPlan p = make_cplan(M);
+    int ns = M;
+    for (int s = 0; s < S; ++s) ns /= p.r[s];
+    return ns;
+}
+
+template <int M, int T, int S>
+__device__ inline void fft_dif_stages_c(double2 *buf, const double2 *__restrict__ roots, double sgn) {
+    constexpr CPlan P = make_cplan(M);
+    if constexpr (S < P.n) {
+        constexpr int p = P.r[S];
+        constexpr int Ns = cplan_ns(M, S);
+        constexpr int q = Ns / p;
+        constexpr int N = 2 * M;
+        constexpr int twm = N / Ns;
+        for (int b = threadIdx.x; b < T * (M / p); b += 256) {
+            const int t = b % T, bb = b / T;
+            const int g = bb / q, n1 = bb - g * q;
+            double2 *a = buf + t * M + g * Ns + n1;
+            dif_butterfly<p>(a, q, n1, twm, roots, N, sgn);
+        }
+        __syncthreads();
+        fft_dif_stages_c<M, T, S + 1>(buf, roots, sgn);
+    }
+}
+
+// MC > 0: length M = MC known at compile time (T latitudes per block, TC); MC == 0: runtime plan.
+template <int MC, int TC>
 __global__ __launch_bounds__(256) void fft_synth_kernel(const double2 *__restrict__ F, long four_dist,
                                                         double *__restrict__ V, long spat_dist,
                                                         const double2 *__restrict__ roots, const int *__restrict__ pos,
-                                                        FftPlan plan, int nlat, int imlim, long sp, long stt, int T) {
+                                                        FftPlan plan, int nlat, int imlim, long sp, long stt, int Trt) {
     extern __shared__ double2 buf[];
-    const int M = plan.M;
-    const int N = 2 * M;
+    const int M = MC ? MC : plan.M;
+    const int T = MC ? TC : Trt;
     const int lat0 = xcd_group(blockIdx.x, gridDim.x) * T;
     const int f = blockIdx.y;
     const double2 *Ff = F + (size_t)f * four_dist;
@@ -717,7 +808,8 @@ __global__ __launch_bounds__(256) void fft_synth_kernel(const double2 *__restric
         buf[(size_t)t * M + k] = Z;
     }
     __syncthreads();
-    fft_dif_inplace(buf, T, M, plan, roots, +1.0);
+    if constexpr (MC > 0) fft_dif_stages_c<MC, TC, 0>(buf, roots, +1.0);
+    else fft_dif_inplace(buf, T, M, plan, roots, +1.0);
     for (int e = threadIdx.x; e < T * M; e += blockDim.x) {
         const int t = e % T, n = e / T;
         if (t < Tv) {
@@ -727,16 +819,17 @@ __global__ __launch_bounds__(256) void fft_synth_kernel(const double2 *__restric
             Vf[(size_t)(2 * n + 1) * sp + (size_t)lat * stt] = z.y;
         }
     }
-    (void)N;
 }
 
+template <int MC, int TC>
 __global__ __launch_bounds__(256) void fft_anal_kernel(const double *__restrict__ V, long spat_dist,
                                                        double2 *__restrict__ F, long four_dist,
                                                        const double2 *__restrict__ roots, const int *__restrict__ pos,
                                                        FftPlan plan, int nlat, int mmax, int imlim, long sp, long stt,
-                                                       int T) {
+                                                       int Trt) {
     extern __shared__ double2 buf[];
-    const int M = plan.M;
+    const int M = MC ? MC : plan.M;
+    const int T = MC ? TC : Trt;
     const int lat0 = xcd_group(blockIdx.x, gridDim.x) * T;
     const int f = blockIdx.y;
     const double *Vf = V + (size_t)f * spat_dist;
@@ -752,7 +845,8 @@ __global__ __launch_bounds__(256) void fft_anal_kernel(const double *__restrict_
         buf[(size_t)t * M + n] = z;
     }
     __syncthreads();
-    fft_dif_inplace(buf, T, M, plan, roots, -1.0);
+    if constexpr (MC > 0) fft_dif_stages_c<MC, TC, 0>(buf, roots, -1.0);
+    else fft_dif_inplace(buf, T, M, plan, roots, -1.0);
     for (int e = threadIdx.x; e < T * (mmax + 1); e += blockDim.x) {
         const int t = e % T, k = e / T;
         if (t >= Tv) continue;
@@ -783,27 +877,60 @@ static int fft_lat_per_block(int M) {
     return T;
 }
 
+// latitudes per block of the specialized kernels: 2 while 2 sequences fit in 64 KB of LDS
+static constexpr int fft_tc(int M) { return (32L * M <= 65536) ? 2 : 1; }
+
+#define FFT_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) \
+    X(3072) X(4096) X(6144)
+
+template <class K>
+static void set_lds_limit(K kern, size_t lds) {
+    if (lds > 65536) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                       const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s) {
     const int M = nphi / 2;
+    static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
+#define FFT_SYN_CASE(MM)                                                                                        \
+    if (!generic && M == MM) {                                                                                  \
+        constexpr int TC = fft_tc(MM);                                                                          \
+        const size_t lds = (size_t)TC * MM * sizeof(double2);                                                   \
+        set_lds_limit(fft_synth_kernel<MM, TC>, lds);                                                           \
+        hipLaunchKernelGGL((fft_synth_kernel<MM, TC>), dim3((nlat + TC - 1) / TC, nfields), dim3(256), lds, s, F, \
+                           four_dist, V, spat_dist, t->roots, t->fft_pos, t->plan, nlat, imlim, sp, st, TC);     \
+        return;                                                                                                 \
+    }
+    FFT_SIZES(FFT_SYN_CASE)
+#undef FFT_SYN_CASE
     const int T = fft_lat_per_block(M);
-    dim3 grid((nlat + T - 1) / T, nfields);
     const size_t lds = (size_t)T * M * sizeof(double2);
-    if (lds > 65536) (void)hipFuncSetAttribute((const void *)fft_synth_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(fft_synth_kernel, grid, dim3(256), lds, s, F, four_dist, V, spat_dist, t->roots, t->fft_pos,
-                       t->plan, nlat, imlim, sp, st, T);
+    set_lds_limit(fft_synth_kernel<0, 1>, lds);
+    hipLaunchKernelGGL((fft_synth_kernel<0, 1>), dim3((nlat + T - 1) / T, nfields), dim3(256), lds, s, F, four_dist, V,
+                       spat_dist, t->roots, t->fft_pos, t->plan, nlat, imlim, sp, st, T);
     (void)mmax;
 }
 
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields, const double *V,
                      long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s) {
     const int M = nphi / 2;
+    static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
+#define FFT_ANA_CASE(MM)                                                                                          \
+    if (!generic && M == MM) {                                                                                    \
+        constexpr int TC = fft_tc(MM);                                                                            \
+        const size_t lds = (size_t)TC * MM * sizeof(double2);                                                     \
+        set_lds_limit(fft_anal_kernel<MM, TC>, lds);                                                              \
+        hipLaunchKernelGGL((fft_anal_kernel<MM, TC>), dim3((nlat + TC - 1) / TC, nfields), dim3(256), lds, s, V,     \
+                           spat_dist, F, four_dist, t->roots, t->fft_pos, t->plan, nlat, mmax, imlim, sp, st, TC); \
+        return;                                                                                                   \
+    }
+    FFT_SIZES(FFT_ANA_CASE)
+#undef FFT_ANA_CASE
     const int T = fft_lat_per_block(M);
-    dim3 grid((nlat + T - 1) / T, nfields);
     const size_t lds = (size_t)T * M * sizeof(double2);
-    if (lds > 65536) (void)hipFuncSetAttribute((const void *)fft_anal_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(fft_anal_kernel, grid, dim3(256), lds, s, V, spat_dist, F, four_dist, t->roots, t->fft_pos,
-                       t->plan, nlat, mmax, imlim, sp, st, T);
+    set_lds_limit(fft_anal_kernel<0, 1>, lds);
+    hipLaunchKernelGGL((fft_anal_kernel<0, 1>), dim3((nlat + T - 1) / T, nfields), dim3(256), lds, s, V, spat_dist, F,
+                       four_dist, t->roots, t->fft_pos, t->plan, nlat, mmax, imlim, sp, st, T);
 }
 
 // ---------------------------------------------------------------------------------------
