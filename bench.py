@@ -76,6 +76,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--layout", choices=["theta", "phi"], default="theta",
                     help="spatial layout (the metric is quoted on theta-contiguous)")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="after the timed loop, copy a 512 MiB buffer once (reference kernel of known "
+                         "traffic for hardware-counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -125,6 +128,14 @@ def main():
     elapsed = t1 - t0
     pairs = BATCH * args.steps
     value = pairs / elapsed
+
+    if args.calib_copy:
+        a = torch.empty(64 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)     # reads 512 MiB, writes 512 MiB
+        torch.cuda.synchronize()
+        del a, b
 
     # accuracy of the timed output (round trip), outside the timed region
     err = float(torch.linalg.norm((q2 - q).flatten()) / torch.linalg.norm(q.flatten()))

@@ -403,7 +403,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     double mpos_scale, int lat_weight) {
     constexpr int NCOL = 16 * NCG;
     __shared__ double ytile[NW][16][65];
-    __shared__ double red[2][NW][16][NCOL + 1];
+    __shared__ double red[NW][16][NCOL + 1];   // single-buffered: two barriers per block, 2 blocks/CU fit
     __shared__ double carry[2][NCOL];
     __shared__ int s_ibeg[NW];
 
@@ -487,14 +487,28 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const int nsteps_out = (spec_ltop >= o.mfun) ? (spec_ltop - o.mfun) / 2 + 1 : 0;
     double2 *Qbase = Q + o.spec_base;
 
+    // post-mix work split: thread handles elements e = threadIdx.x + j*64*NW (j < NE) of the
+    // 16 x NCOL block; their coefficient rows are prefetched before the MFMA phase.
+    constexpr int NE = (16 * NCOL + 64 * NW - 1) / (64 * NW);
     int b = 0;
     for (int i = ibeg; i < nsteps; i += 16, ++b) {
+        double pcx[NE], pcz[NE], pcp[NE];
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            const int e = threadIdx.x + j * 64 * NW;
+            const int row = min(e / NCOL, 15);
+            const int ii = min(i + row, nsteps - 1);
+            const double4 cf4 = pmm[ii];
+            pcx[j] = cf4.x;
+            pcz[j] = cf4.z;
+            pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
+        }
         double2 cf[16];
 #pragma unroll
         for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
-        double4_t acc[NCG];
+        double4_t acc[NCG], acc2[NCG];   // two chains per column group (even / odd K-groups)
 #pragma unroll
-        for (int c = 0; c < NCG; ++c) acc[c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+        for (int c = 0; c < NCG; ++c) acc[c] = acc2[c] = (double4_t){0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int r = 0; r < NLR; ++r) {
             if (i <= my_imax) {   // some lane of this wave starts its recurrence here
@@ -518,47 +532,56 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             }
             __builtin_amdgcn_s_waitcnt(0xc07f);
             __builtin_amdgcn_wave_barrier();
+            double a[16];
 #pragma unroll
-            for (int kg = 0; kg < 16; ++kg) {
-                const double a = ytile[wave][lane & 15][4 * kg + (lane >> 4)];
-#pragma unroll
-                for (int c = 0; c < NCG; ++c) acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, Ereg[r][c][kg], acc[c], 0, 0, 0);
-            }
+            for (int kg = 0; kg < 16; ++kg) a[kg] = ytile[wave][lane & 15][4 * kg + (lane >> 4)];
+            __builtin_amdgcn_s_waitcnt(0xc07f);   // all A operands in registers before the tile is rewritten
             __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int kg = 0; kg < 16; kg += 2)
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) {
+                    acc[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg], Ereg[r][c][kg], acc[c], 0, 0, 0);
+                    acc2[c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kg + 1], Ereg[r][c][kg + 1], acc2[c], 0, 0, 0);
+                }
         }
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) acc[c] += acc2[c];
         // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
         const int pb = b & 1;
+        __syncthreads();   // previous block's post-mix has finished reading red
 #pragma unroll
         for (int c = 0; c < NCG; ++c)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) red[pb][wave][4 * j + (lane >> 4)][16 * c + (lane & 15)] = acc[c][j];
+            for (int j = 0; j < 4; ++j) red[wave][4 * j + (lane >> 4)][16 * c + (lane & 15)] = acc[c][j];
         __syncthreads();
-        // sum over waves + post-mix; thread e handles (step row, column)
-        for (int e = threadIdx.x; e < 16 * NCOL; e += 64 * NW) {
+        // sum over waves + post-mix
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            const int e = threadIdx.x + j * 64 * NW;
+            if (e >= 16 * NCOL) break;
             const int row = e / NCOL, col = e - row * NCOL;
             const int comp = col & 3;
             const int f = fbase + (col >> 2);
             double sum = 0.0, sprev = 0.0;
 #pragma unroll
-            for (int w = 0; w < NW; ++w) sum += red[pb][w][row][col];
+            for (int w = 0; w < NW; ++w) sum += red[w][row][col];
             if (row == 15 && comp < 2) carry[pb][col] = sum;
             const int ii = i + row;
             if (f >= nfields || ii >= nsteps_out) continue;
             const int le = o.mfun + 2 * ii;
             double v = 0.0;
             if (ii < nsteps) {
-                const double4 cf = pmm[ii];
                 if (comp < 2) {
                     if (row > 0) {
 #pragma unroll
-                        for (int w = 0; w < NW; ++w) sprev += red[pb][w][row - 1][col];
+                        for (int w = 0; w < NW; ++w) sprev += red[w][row - 1][col];
                     } else {
                         sprev = carry[pb ^ 1][col];
                     }
-                    const double cprev = (ii > 0) ? pmm[ii - 1].y : 0.0;
-                    v = cf.x * sum + cprev * sprev;
+                    v = pcx[j] * sum + pcp[j] * sprev;
                 } else {
-                    v = cf.z * sum;
+                    v = pcz[j] * sum;
                 }
             }
             const int ld = (comp < 2) ? le : le + 1;
@@ -598,12 +621,15 @@ static void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dis
 void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                           int spec_ltop, hipStream_t s) {
     const int ntiles = (a.nlat_2 + 63) / 64;
-    // waves per block: a divisor of the tile count, at most 8; tiles per wave at most 6
+    // waves per block: one latitude tile per wave up to 12 waves (measured fastest: lowest
+    // register count, 3 waves per SIMD); beyond, a divisor of the tile count, at most 8
     int nw = 1;
-    for (int d = 8; d >= 1; --d)
-        if (ntiles % d == 0) { nw = d; break; }
+    if (ntiles <= 12) nw = ntiles;
+    else
+        for (int d = 8; d >= 1; --d)
+            if (ntiles % d == 0) { nw = d; break; }
     static const int nw_env = getenv("SHTNS_AMD_ANAL_NW") ? atoi(getenv("SHTNS_AMD_ANAL_NW")) : 0;
-    if (nw_env > 0 && nw_env <= 8 && ntiles % nw_env == 0) nw = nw_env;
+    if (nw_env > 0 && nw_env <= 16 && ntiles % nw_env == 0) nw = nw_env;
     int nlr = ntiles / nw;
     if (nlr > 6) { nw = 8; nlr = (ntiles + 7) / 8; }
     static const int ncg_env = getenv("SHTNS_AMD_ANAL_NCG") ? atoi(getenv("SHTNS_AMD_ANAL_NCG")) : 0;
@@ -624,6 +650,7 @@ void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dis
     ANAL_CASE(6, 1) ANAL_CASE(6, 2) ANAL_CASE(6, 3) ANAL_CASE(6, 6)
     ANAL_CASE(7, 1) ANAL_CASE(7, 3) ANAL_CASE(7, 6)
     ANAL_CASE(8, 1) ANAL_CASE(8, 2) ANAL_CASE(8, 3) ANAL_CASE(8, 4) ANAL_CASE(8, 6)
+    ANAL_CASE(9, 1) ANAL_CASE(10, 1) ANAL_CASE(11, 1) ANAL_CASE(12, 1)
 #undef ANAL_CASE
     fprintf(stderr, "shtns_amd: no analysis kernel for nlat_2=%d\n", a.nlat_2);
 }
