@@ -15,6 +15,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
 #include <vector>
 
@@ -218,6 +219,10 @@ static void free_device(shtns_info *c) {
     if (c->own_streams) {
         if (c->comp_stream) (void)hipStreamDestroy(c->comp_stream);
     }
+    if (c->aux_stream) (void)hipStreamDestroy(c->aux_stream);
+    c->aux_stream = nullptr;
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+    c->ev_pool.clear();
     c->d = DeviceTables();
     c->scratch = nullptr;
     c->scratch_bytes = 0;
@@ -608,6 +613,10 @@ extern "C" shtns_cfg cushtns_clone(shtns_cfg c, hipStream_t compute_stream, hipS
     k->own_streams = false;
     k->comp_stream = compute_stream;
     k->xfer_stream = transfer_stream;
+    k->aux_stream = nullptr;
+    k->ev_pool.clear();
+    for (auto &v : k->prof_ev) v.clear();
+    k->prof = false;
     std::lock_guard<std::mutex> g(g_mutex);
     k->next = g_cfg_list;
     g_cfg_list = k;
@@ -681,18 +690,41 @@ enum { PK_LEG_SYNTH = 0, PK_LEG_ANAL = 1, PK_FFT_SYNTH = 2, PK_FFT_ANAL = 3, PK_
 struct ProfScope {
     shtns_cfg c;
     int k;
+    hipStream_t s;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(shtns_cfg c_, int k_) : c(c_), k(k_) {
+    ProfScope(shtns_cfg c_, int k_, hipStream_t s_ = nullptr, bool use_s = false) : c(c_), k(k_), s(use_s ? s_ : c_->comp_stream) {
         if (!c->prof) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
-        (void)hipEventRecord(a, c->comp_stream);
+        (void)hipEventRecord(a, s);
     }
     ~ProfScope() {
         if (!a) return;
-        (void)hipEventRecord(b, c->comp_stream);
+        (void)hipEventRecord(b, s);
         c->prof_ev[k].emplace_back(a, b);
     }
 };
+
+// second stream + an event pool for the overlapped (chunked) transforms
+static hipStream_t aux_stream(shtns_cfg c) {
+    if (!c->aux_stream) {
+        if (hipStreamCreateWithFlags(&c->aux_stream, hipStreamNonBlocking) != hipSuccess) c->aux_stream = nullptr;
+    }
+    return c->aux_stream;
+}
+static hipEvent_t pool_event(shtns_cfg c, size_t i) {
+    while (c->ev_pool.size() <= i) {
+        hipEvent_t e;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[i];
+}
+// Overlapping measured slower at lmax=1023 (the analysis blocks use 150 KB of LDS and cannot
+// share a CU with FFT blocks), so it is opt-in: SHTNS_AMD_OVERLAP=1.
+static bool overlap_enabled() {
+    static const bool on = getenv("SHTNS_AMD_OVERLAP") != nullptr;
+    return on;
+}
 
 static void prof_clear(shtns_cfg c) {
     for (auto &v : c->prof_ev) {
@@ -728,6 +760,9 @@ extern "C" int shtns_profile_get(shtns_cfg c, double *ms, int *count) {
 }
 
 // synthesis core: Q (nfields, spec_dist) -> V (nfields, spat_dist)
+// The Legendre kernel runs in chunks of latitude tiles on the compute stream; the FFT of a
+// chunk's latitudes (north and south rows) runs on the second stream as soon as the chunk
+// is done, overlapping the next chunk's Legendre work (matrix cores vs LDS/HBM).
 static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
                        double mpos_scale, int lat_weight) {
     if (nfields <= 0) return;
@@ -743,21 +778,51 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     LegendreArgs a = leg_args(c, ltr, imlim, nfields);
     a.mpos_scale = mpos_scale;
     a.lat_weight = lat_weight;
-    {
-        ProfScope ps(c, PK_LEG_SYNTH);
-        launch_legendre_synth(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, F, four_dist, qmix,
-                              c->nsteps_total, c->comp_stream);
-    }
     long sp, st;
     spat_strides(c, sp, st);
+    const hipStream_t S = c->comp_stream;
+    const int ntiles = (c->nlat_2 + 63) / 64;
+    const int chunk_tiles = 4;
+    const int nchunks = (ntiles + chunk_tiles - 1) / chunk_tiles;
+    hipStream_t A = (overlap_enabled() && nchunks > 1) ? aux_stream(c) : nullptr;
     {
-        ProfScope ps(c, PK_FFT_SYNTH);
-        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st,
-                         c->comp_stream);
+        ProfScope ps(c, PK_LEG_SYNTH);
+        launch_premix(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, qmix, c->nsteps_total, S);
     }
+    if (!A) {
+        {
+            ProfScope ps(c, PK_LEG_SYNTH);
+            launch_legendre_synth_tiles(a, c->lmax, F, four_dist, qmix, c->nsteps_total, 0, ntiles, S);
+        }
+        ProfScope ps(c, PK_FFT_SYNTH);
+        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S);
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int t0 = ch * chunk_tiles, ntl = std::min(chunk_tiles, ntiles - t0);
+        {
+            ProfScope ps(c, PK_LEG_SYNTH);
+            launch_legendre_synth_tiles(a, c->lmax, F, four_dist, qmix, c->nsteps_total, t0, ntl, S);
+        }
+        hipEvent_t ev = pool_event(c, ch);
+        HIP_OK(hipEventRecord(ev, S));
+        HIP_OK(hipStreamWaitEvent(A, ev, 0));
+        const int k0 = 64 * t0, k1 = std::min(64 * (t0 + ntl), (int)c->nlat_2);
+        ProfScope ps(c, PK_FFT_SYNTH, A, true);
+        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, A, k0,
+                         k1 - k0);
+        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, A,
+                         c->nlat - k1, k1 - k0);
+    }
+    hipEvent_t done = pool_event(c, nchunks);
+    HIP_OK(hipEventRecord(done, A));
+    HIP_OK(hipStreamWaitEvent(S, done, 0));
     HIP_OK(hipGetLastError());
 }
 
+// analysis core: V -> Q.  With 16+ fields the FFTs of 8-field chunks run on the second
+// stream ahead of the Legendre analysis of the previous chunk on the compute stream.
 static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
                       double mpos_scale, int lat_weight) {
     if (nfields <= 0) return;
@@ -768,17 +833,42 @@ static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfield
     if (!F) return;
     long sp, st;
     spat_strides(c, sp, st);
-    {
-        ProfScope ps(c, PK_FFT_ANAL);
-        launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st,
-                        c->comp_stream);
-    }
-    LegendreArgs a = leg_args(c, ltr, imlim, nfields);
-    a.mpos_scale = mpos_scale;
-    a.lat_weight = lat_weight;
-    {
+    const hipStream_t S = c->comp_stream;
+    const int chunk = 8;
+    const int nchunks = (nfields + chunk - 1) / chunk;
+    hipStream_t A = (overlap_enabled() && nchunks > 1) ? aux_stream(c) : nullptr;
+    if (!A) {
+        {
+            ProfScope ps(c, PK_FFT_ANAL);
+            launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, S);
+        }
+        LegendreArgs a = leg_args(c, ltr, imlim, nfields);
+        a.mpos_scale = mpos_scale;
+        a.lat_weight = lat_weight;
         ProfScope ps(c, PK_LEG_ANAL);
-        launch_legendre_anal(a, F, four_dist, reinterpret_cast<double2 *>(Q), spec_dist, c->lmax, c->comp_stream);
+        launch_legendre_anal(a, F, four_dist, reinterpret_cast<double2 *>(Q), spec_dist, c->lmax, S);
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    hipEvent_t ready = pool_event(c, 0);
+    HIP_OK(hipEventRecord(ready, S));
+    HIP_OK(hipStreamWaitEvent(A, ready, 0));
+    for (int ch = 0; ch < nchunks; ++ch) {
+        const int f0 = ch * chunk, nf = std::min(chunk, nfields - f0);
+        {
+            ProfScope ps(c, PK_FFT_ANAL, A, true);
+            launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf, V + (size_t)f0 * spat_dist, spat_dist,
+                            F + (size_t)f0 * four_dist, four_dist, sp, st, A);
+        }
+        hipEvent_t ev = pool_event(c, 1 + ch);
+        HIP_OK(hipEventRecord(ev, A));
+        HIP_OK(hipStreamWaitEvent(S, ev, 0));
+        LegendreArgs a = leg_args(c, ltr, imlim, nf);
+        a.mpos_scale = mpos_scale;
+        a.lat_weight = lat_weight;
+        ProfScope ps(c, PK_LEG_ANAL);
+        launch_legendre_anal(a, F + (size_t)f0 * four_dist, four_dist, reinterpret_cast<double2 *>(Q) + (size_t)f0 * spec_dist,
+                             spec_dist, c->lmax, S);
     }
     HIP_OK(hipGetLastError());
 }

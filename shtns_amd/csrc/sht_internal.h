@@ -79,6 +79,10 @@ struct shtns_info {
     size_t scratch_bytes = 0;
     shtns_info *next = nullptr;
     shtns_info *clone_of = nullptr;   // tables borrowed from this config
+    // second stream and events used to overlap FFT kernels with Legendre kernels of another
+    // chunk (created on first use; each clone has its own)
+    hipStream_t aux_stream = nullptr;
+    std::vector<hipEvent_t> ev_pool;
     // optional per-kernel timing with HIP events on the compute stream (shtns_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[8];
@@ -104,10 +108,20 @@ size_t synth_qmix_doubles(int nfields, int nsteps_total);
 void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dist,
                           double2 *Qlm, long spec_dist, int spec_ltop, hipStream_t s);
 // Spatial element (ip, it) of a field sits at ip*sp + it*st (theta-contiguous: sp=nlat, st=1).
+// lat_base / nlat_chunk restrict the transform to latitudes [lat_base, lat_base + nlat_chunk)
+// (nlat_chunk < 0: all latitudes).
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
-                      const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s);
+                      const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s,
+                      int lat_base = 0, int nlat_chunk = -1);
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
-                     const double *V, long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s);
+                     const double *V, long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s,
+                     int lat_base = 0, int nlat_chunk = -1);
+// the two halves of launch_legendre_synth: coefficient pre-mix, then the Legendre kernel for
+// latitude tiles [tile0, tile0 + ntl) (64 latitude pairs per tile)
+void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop, double *qmix,
+                   int nsteps_total, hipStream_t s);
+void launch_legendre_synth_tiles(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
+                                 int nsteps_total, int tile0, int ntl, hipStream_t s);
 void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s);
 void launch_sphtor_to_vw(const double2 *S, const double2 *T, long spec_dist, double2 *VW, long vw_dist,
                          int lmax, int mmax, int mres, int ltr, int imlim, int nfields, hipStream_t s);

@@ -216,19 +216,19 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
     const int *__restrict__ i0tab, const double2 *__restrict__ z0tab, const double *__restrict__ qmix,
     int nsteps_total, int lmax_spec, double2 *__restrict__ F, long four_dist, int mmax, int mres, int nlat,
-    int nlat_2, int ltr, int imlim, int nfields, int vector_mode, int lat_weight, int ntiles) {
+    int nlat_2, int ltr, int imlim, int nfields, int vector_mode, int lat_weight, int tile0, int ntl) {
     constexpr int NCOL = 16 * NCG;
     __shared__ double ytile[kSynthWaves][2][4][64];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     // heavy orders first: block x -> (order, tile group); all tile groups of an order adjacent
-    const int ntg = (ntiles + kSynthWaves - 1) / kSynthWaves;
+    const int ntg = (ntl + kSynthWaves - 1) / kSynthWaves;   // tile groups of this launch
     const int im = blockIdx.x / ntg;
-    const int tile = (blockIdx.x - im * ntg) * kSynthWaves + wave;
+    const int tile = tile0 + (blockIdx.x - im * ntg) * kSynthWaves + wave;
     const int g = blockIdx.y;
     const int fbase = g * 4 * NCG;
-    if (tile >= ntiles) return;   // no block-level synchronization in this kernel
+    if (tile >= tile0 + ntl || tile * 64 >= nlat_2) return;   // no block-level synchronization in this kernel
     const int k = tile * 64 + lane;
     const bool kvalid = (k < nlat_2);
     const int kk = kvalid ? k : (nlat_2 - 1);
@@ -357,28 +357,37 @@ size_t synth_qmix_doubles(int nfields, int nsteps_total) {
     return (size_t)ng * nsteps_total * 16 * ncg;
 }
 
-void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
-                           double2 *F, long four_dist, double *qmix, int nsteps_total, hipStream_t s) {
+void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop, double *qmix,
+                   int nsteps_total, hipStream_t s) {
     const DeviceTables *t = a.t;
-    const int ntiles = (a.nlat_2 + 63) / 64;
-    const int ntg = (ntiles + kSynthWaves - 1) / kSynthWaves;
     const int ncg = synth_ncg(a.nfields);
     const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
-    {
-        dim3 grid(a.mmax + 1, ng, 4);
-        hipLaunchKernelGGL(premix_kernel, grid, dim3(256), 0, s, t->step_ofs, t->pm, Qlm, spec_dist, spec_ltop, qmix,
-                           nsteps_total, 16 * ncg, a.mmax, a.mres, a.ltr, a.imlim, a.nfields, a.vector_mode,
-                           a.mpos_scale);
-    }
+    dim3 grid(a.mmax + 1, ng, 4);
+    hipLaunchKernelGGL(premix_kernel, grid, dim3(256), 0, s, t->step_ofs, t->pm, Qlm, spec_dist, spec_ltop, qmix,
+                       nsteps_total, 16 * ncg, a.mmax, a.mres, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale);
+}
+
+void launch_legendre_synth_tiles(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
+                                 int nsteps_total, int tile0, int ntl, hipStream_t s) {
+    const DeviceTables *t = a.t;
+    const int ntg = (ntl + kSynthWaves - 1) / kSynthWaves;
+    const int ncg = synth_ncg(a.nfields);
+    const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
     dim3 grid((a.mmax + 1) * ntg, ng);
 #define SYNTH_LAUNCH(NC)                                                                                         \
     hipLaunchKernelGGL((legendre_synth_kernel<NC>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq, \
                        t->step_ofs, t->ab, t->i0, t->z0, qmix, nsteps_total, spec_ltop, F, four_dist, a.mmax,    \
-                       a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.lat_weight, ntiles)
+                       a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.lat_weight, tile0, ntl)
     if (ncg == 4) SYNTH_LAUNCH(4);
     else if (ncg == 2) SYNTH_LAUNCH(2);
     else SYNTH_LAUNCH(1);
 #undef SYNTH_LAUNCH
+}
+
+void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
+                           double2 *F, long four_dist, double *qmix, int nsteps_total, hipStream_t s) {
+    launch_premix(a, Qlm, spec_dist, spec_ltop, qmix, nsteps_total, s);
+    launch_legendre_synth_tiles(a, spec_ltop, F, four_dist, qmix, nsteps_total, 0, (a.nlat_2 + 63) / 64, s);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -809,15 +818,16 @@ template <int MC, int TC>
 __global__ __launch_bounds__(256) void fft_synth_kernel(const double2 *__restrict__ F, long four_dist,
                                                         double *__restrict__ V, long spat_dist,
                                                         const double2 *__restrict__ roots, const int *__restrict__ pos,
-                                                        FftPlan plan, int nlat, int imlim, long sp, long stt, int Trt) {
+                                                        FftPlan plan, int nlat, int imlim, long sp, long stt, int Trt,
+                                                        int lat_base, int lat_end) {
     extern __shared__ double2 buf[];
     const int M = MC ? MC : plan.M;
     const int T = MC ? TC : Trt;
-    const int lat0 = xcd_group(blockIdx.x, gridDim.x) * T;
+    const int lat0 = lat_base + xcd_group(blockIdx.x, gridDim.x) * T;
     const int f = blockIdx.y;
     const double2 *Ff = F + (size_t)f * four_dist;
     double *Vf = V + (size_t)f * spat_dist;
-    const int Tv = min(T, nlat - lat0);
+    const int Tv = min(T, lat_end - lat0);
     // build Z (natural order)
     for (int e = threadIdx.x; e < T * M; e += blockDim.x) {
         const int t = e % T, k = e / T;
@@ -853,15 +863,15 @@ __global__ __launch_bounds__(256) void fft_anal_kernel(const double *__restrict_
                                                        double2 *__restrict__ F, long four_dist,
                                                        const double2 *__restrict__ roots, const int *__restrict__ pos,
                                                        FftPlan plan, int nlat, int mmax, int imlim, long sp, long stt,
-                                                       int Trt) {
+                                                       int Trt, int lat_base, int lat_end) {
     extern __shared__ double2 buf[];
     const int M = MC ? MC : plan.M;
     const int T = MC ? TC : Trt;
-    const int lat0 = xcd_group(blockIdx.x, gridDim.x) * T;
+    const int lat0 = lat_base + xcd_group(blockIdx.x, gridDim.x) * T;
     const int f = blockIdx.y;
     const double *Vf = V + (size_t)f * spat_dist;
     double2 *Ff = F + (size_t)f * four_dist;
-    const int Tv = min(T, nlat - lat0);
+    const int Tv = min(T, lat_end - lat0);
     for (int e = threadIdx.x; e < T * M; e += blockDim.x) {
         const int t = e % T, n = e / T;
         double2 z = make_double2(0.0, 0.0);
@@ -916,16 +926,20 @@ static void set_lds_limit(K kern, size_t lds) {
 }
 
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
-                      const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s) {
+                      const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s,
+                      int lat_base, int nlat_chunk) {
     const int M = nphi / 2;
+    if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
+    const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
 #define FFT_SYN_CASE(MM)                                                                                        \
     if (!generic && M == MM) {                                                                                  \
         constexpr int TC = fft_tc(MM);                                                                          \
         const size_t lds = (size_t)TC * MM * sizeof(double2);                                                   \
         set_lds_limit(fft_synth_kernel<MM, TC>, lds);                                                           \
-        hipLaunchKernelGGL((fft_synth_kernel<MM, TC>), dim3((nlat + TC - 1) / TC, nfields), dim3(256), lds, s, F, \
-                           four_dist, V, spat_dist, t->roots, t->fft_pos, t->plan, nlat, imlim, sp, st, TC);     \
+        hipLaunchKernelGGL((fft_synth_kernel<MM, TC>), dim3((nlat_chunk + TC - 1) / TC, nfields), dim3(256), lds, s, \
+                           F, four_dist, V, spat_dist, t->roots, t->fft_pos, t->plan, nlat, imlim, sp, st, TC,  \
+                           lat_base, lat_end);                                                                  \
         return;                                                                                                 \
     }
     FFT_SIZES(FFT_SYN_CASE)
@@ -933,22 +947,26 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
     const int T = fft_lat_per_block(M);
     const size_t lds = (size_t)T * M * sizeof(double2);
     set_lds_limit(fft_synth_kernel<0, 1>, lds);
-    hipLaunchKernelGGL((fft_synth_kernel<0, 1>), dim3((nlat + T - 1) / T, nfields), dim3(256), lds, s, F, four_dist, V,
-                       spat_dist, t->roots, t->fft_pos, t->plan, nlat, imlim, sp, st, T);
+    hipLaunchKernelGGL((fft_synth_kernel<0, 1>), dim3((nlat_chunk + T - 1) / T, nfields), dim3(256), lds, s, F,
+                       four_dist, V, spat_dist, t->roots, t->fft_pos, t->plan, nlat, imlim, sp, st, T, lat_base, lat_end);
     (void)mmax;
 }
 
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields, const double *V,
-                     long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s) {
+                     long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
+                     int nlat_chunk) {
     const int M = nphi / 2;
+    if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
+    const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
 #define FFT_ANA_CASE(MM)                                                                                          \
     if (!generic && M == MM) {                                                                                    \
         constexpr int TC = fft_tc(MM);                                                                            \
         const size_t lds = (size_t)TC * MM * sizeof(double2);                                                     \
         set_lds_limit(fft_anal_kernel<MM, TC>, lds);                                                              \
-        hipLaunchKernelGGL((fft_anal_kernel<MM, TC>), dim3((nlat + TC - 1) / TC, nfields), dim3(256), lds, s, V,     \
-                           spat_dist, F, four_dist, t->roots, t->fft_pos, t->plan, nlat, mmax, imlim, sp, st, TC); \
+        hipLaunchKernelGGL((fft_anal_kernel<MM, TC>), dim3((nlat_chunk + TC - 1) / TC, nfields), dim3(256), lds, s,   \
+                           V, spat_dist, F, four_dist, t->roots, t->fft_pos, t->plan, nlat, mmax, imlim, sp, st, TC, \
+                           lat_base, lat_end);                                                                     \
         return;                                                                                                   \
     }
     FFT_SIZES(FFT_ANA_CASE)
@@ -956,8 +974,9 @@ void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int im
     const int T = fft_lat_per_block(M);
     const size_t lds = (size_t)T * M * sizeof(double2);
     set_lds_limit(fft_anal_kernel<0, 1>, lds);
-    hipLaunchKernelGGL((fft_anal_kernel<0, 1>), dim3((nlat + T - 1) / T, nfields), dim3(256), lds, s, V, spat_dist, F,
-                       four_dist, t->roots, t->fft_pos, t->plan, nlat, mmax, imlim, sp, st, T);
+    hipLaunchKernelGGL((fft_anal_kernel<0, 1>), dim3((nlat_chunk + T - 1) / T, nfields), dim3(256), lds, s, V,
+                       spat_dist, F, four_dist, t->roots, t->fft_pos, t->plan, nlat, mmax, imlim, sp, st, T, lat_base,
+                       lat_end);
 }
 
 // ---------------------------------------------------------------------------------------
