@@ -40,6 +40,29 @@ def fft_flops(nlat, nphi):
     return 2.5 * nlat * nphi * math.log2(nphi)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sht_pair_l1023_b16_hbm_traffic.txt")
+TRAFFIC_ROWS = {"legendre_synth": ("premix_kernel", "legendre_synth_kernel"),
+                "legendre_anal": ("legendre_anal_kernel",), "fft_synth": ("fft_synth",), "fft_anal": ("fft_anal",)}
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the copy-kernel
+    calibration) from the committed rocprofv3 --pmc summary of this workload, or None."""
+    try:
+        lines = open(TRAFFIC_PROFILE).read().splitlines()
+    except OSError:
+        return None
+    total, found = 0.0, 0
+    for prefix in TRAFFIC_ROWS.get(kernel, ()):
+        for ln in lines:
+            if ln.startswith(prefix):
+                parts = ln.split()
+                total += (float(parts[-2]) + float(parts[-1])) * 1e6
+                found += 1
+                break
+    return total if found == len(TRAFFIC_ROWS.get(kernel, ())) and found else None
+
+
 def cpu_baseline(budget_s=10.0):
     """Time the CPU oracle on one-field SH_to_spat + spat_to_SH pairs of the workload."""
     from oracle.oracle import Oracle, set_threads
@@ -73,6 +96,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-prec", action="store_true", help="skip the FP32 / FP32-split contraction legs")
     ap.add_argument("--cpu-budget", type=float, default=10.0)
     ap.add_argument("--layout", choices=["theta", "phi"], default="theta",
                     help="spatial layout (the metric is quoted on theta-contiguous)")
@@ -109,24 +133,43 @@ def main():
         lib.cu_SH_to_spat_batch(s._cfg, qp, vp, LMAX, BATCH, nlm, nspat)
         lib.cu_spat_to_SH_batch(s._cfg, vp, q2p, LMAX, BATCH, nspat, nlm)
 
-    for _ in range(args.warmup):
-        step()
-    _lib.check_error("warmup")
-    torch.cuda.synchronize()
-    lib.shtns_profile(s._cfg, 1)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    _lib.check_error("timed loop")
-    ms = (ctypes.c_double * 8)()
-    cnt = (ctypes.c_int * 8)()
-    lib.shtns_profile_get(s._cfg, ms, cnt)
-    lib.shtns_profile(s._cfg, 0)
-    elapsed = t1 - t0
+    names = ["legendre_synth", "legendre_anal", "fft_synth", "fft_anal"]
+    flops = [leg_flops(LMAX, NLAT) * BATCH, leg_flops(LMAX, NLAT) * BATCH,
+             fft_flops(NLAT, NPHI) * BATCH, fft_flops(NLAT, NPHI) * BATCH]
+
+    def timed_loop(prec):
+        """W untimed steps, then K steps between two device synchronizations; the library's
+        HIP events on its compute stream time each kernel inside the timed region."""
+        lib.cushtns_set_precision(s._cfg, prec)
+        for _ in range(args.warmup):
+            step()
+        _lib.check_error("warmup")
+        torch.cuda.synchronize()
+        lib.shtns_profile(s._cfg, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        _lib.check_error("timed loop")
+        ms = (ctypes.c_double * 8)()
+        cnt = (ctypes.c_int * 8)()
+        lib.shtns_profile_get(s._cfg, ms, cnt)
+        lib.shtns_profile(s._cfg, 0)
+        elapsed = t1 - t0
+        # One transform call may launch a kernel several times (premix + Legendre, or chunks);
+        # times are summed per call: per_call_ms = total / steps.
+        kern = {}
+        for k, nm in enumerate(names):
+            if cnt[k]:
+                per_call = ms[k] / args.steps * 1e-3
+                kern[nm] = {"per_call_ms": per_call * 1e3, "launches_per_call": cnt[k] / args.steps,
+                            "share": ms[k] * 1e-3 / elapsed, "tflops": flops[k] / per_call / 1e12}
+        return elapsed, kern
+
     pairs = BATCH * args.steps
+    elapsed, kern = timed_loop(-1)
     value = pairs / elapsed
 
     if args.calib_copy:
@@ -140,24 +183,32 @@ def main():
     # accuracy of the timed output (round trip), outside the timed region
     err = float(torch.linalg.norm((q2 - q).flatten()) / torch.linalg.norm(q.flatten()))
 
-    names = ["legendre_synth", "legendre_anal", "fft_synth", "fft_anal"]
-    flops = [leg_flops(LMAX, NLAT) * BATCH, leg_flops(LMAX, NLAT) * BATCH,
-             fft_flops(NLAT, NPHI) * BATCH, fft_flops(NLAT, NPHI) * BATCH]
-    # One transform call may launch a kernel several times (chunks overlapped with the FFT on a
-    # second stream); times are summed per call: per_call_ms = total / steps.
-    kern = {}
-    for k, nm in enumerate(names):
-        if cnt[k]:
-            per_call = ms[k] / args.steps * 1e-3
-            kern[nm] = {"per_call_ms": per_call * 1e3, "launches_per_call": cnt[k] / args.steps,
-                        "share": ms[k] * 1e-3 / elapsed, "tflops": flops[k] / per_call / 1e12}
+    # reduced-precision contractions on the same inputs and timing protocol (recurrence,
+    # seeds, FFT and final sums stay FP64), errors against the FP64 outputs above
+    prec_legs = {}
+    if not args.no_prec:
+        v64 = v.clone()
+        q64 = q2.clone()
+        for name, code in (("fp32_split", 2), ("fp32", 1)):
+            el, kp = timed_loop(code)
+            ev = float(torch.linalg.norm((v - v64).flatten()) / torch.linalg.norm(v64.flatten()))
+            eq = float(torch.linalg.norm((q2 - q64).flatten()) / torch.linalg.norm(q64.flatten()))
+            pv = pairs / el
+            prec_legs[name] = {"pairs_per_s": round(pv, 2), "speedup_vs_fp64": round(pv / value, 3),
+                               "synth_rel_l2_vs_fp64": ev, "pair_rel_l2_vs_fp64": eq,
+                               "legendre_ms": {k: round(kp[k]["per_call_ms"], 4) for k in ("legendre_synth", "legendre_anal")
+                                               if k in kp}}
+        lib.cushtns_set_precision(s._cfg, -1)
+
     dom = max(kern, key=lambda n: kern[n]["share"])
     dk = kern[dom]
+    traffic = profiled_traffic(dom) if args.layout == "theta" else None
     roof = {"bound": "mfma", "achieved": round(dk["tflops"], 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(dk["tflops"] / FP64_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom}
+            "frac": round(dk["tflops"] / FP64_PEAK_TFLOPS, 4), "traffic": traffic, "kernel": dom,
+            "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT) if traffic else None}
     alg_total = (leg_flops(LMAX, NLAT) + fft_flops(NLAT, NPHI)) * 2 * pairs
     out = {
-        "metric": "SH_to_spat+spat_to_SH pairs/s at lmax=1023 (FP64)",
+        "metric": "SH_to_spat+spat_to_SH pairs/s at lmax=1023 (FP32-split vs FP64); % FP64 peak",
         "value": round(value, 2),
         "unit": "pairs/s",
         "n_gpus": 1,
@@ -177,6 +228,7 @@ def main():
         "kernels": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv for kk, vv in d.items()}
                     for k, d in kern.items()},
         "roofline": roof,
+        "precision_legs": prec_legs,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_budget)

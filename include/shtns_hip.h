@@ -22,7 +22,9 @@ extern "C" {
 
 /* ---- transforms on device memory (shtns_cuda.h:38-52) ---- */
 void cu_spat_to_SH(shtns_cfg shtns, double *Vr, cplx *Qlm, int ltr);                       /* :38 */
+void cu_spat_to_SH_float(shtns_cfg shtns, float *Vr, cplx_f *Qlm, int ltr);                /* :40 */
 void cu_SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr, int ltr);                       /* :42 */
+void cu_SH_to_spat_float(shtns_cfg shtns, cplx_f *Qlm, float *Vr, int ltr);                /* :44 */
 void cu_spat_to_SHsphtor(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr); /* :46 */
 void cu_SHsphtor_to_spat(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp, int ltr); /* :48 */
 
@@ -41,6 +43,28 @@ void cu_SHsphtor_to_spat_batch(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, doub
                                int nfields, long spec_dist, long spat_dist);
 void cu_spat_to_SHsphtor_batch(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr,
                                int nfields, long spat_dist, long spec_dist);
+
+/* Single-precision data, batched (additive; the unbatched _float calls above are SHTns's).
+ * Arrays are FP32; the recurrence, seeds and Fourier transform run in FP64 and the
+ * degree/latitude contraction at the configuration's precision (cushtns_set_precision;
+ * default SHT_PREC_F32 for these calls). */
+void cu_SH_to_spat_float_batch(shtns_cfg, cplx_f *Qlm, float *Vr, int ltr, int nfields, long spec_dist, long spat_dist);
+void cu_spat_to_SH_float_batch(shtns_cfg, float *Vr, cplx_f *Qlm, int ltr, int nfields, long spat_dist, long spec_dist);
+
+/* Precision of the Legendre contraction (additive, no SHTns counterpart).  The recurrence,
+ * its seeds and tables, the Gauss nodes and weights, the FFT and the final sums stay FP64.
+ *   SHT_PREC_F64    FP64 matrix cores (default for double data);
+ *   SHT_PREC_F32    recurrence values and coefficients rounded to FP32, FP32 matrix cores,
+ *                   FP32 partial sums over 16 degree pairs accumulated in FP64 (default for
+ *                   float data);
+ *   SHT_PREC_SPLIT  both operands as FP32 hi+lo pairs, three FP32 matrix-core products
+ *                   (hi*hi + lo*hi + hi*lo), same accumulation.
+ * SHT_PREC_AUTO restores the defaults.  Returns 0, or -1 for an unknown value. */
+#define SHT_PREC_AUTO (-1)
+#define SHT_PREC_F64 0
+#define SHT_PREC_F32 1
+#define SHT_PREC_SPLIT 2
+int cushtns_set_precision(shtns_cfg, int prec);
 
 /* Adjoint (transpose) operators, for reverse-mode differentiation (additive).
  *   cu_SH_to_spat_adjoint:  Qlm = S^T(Vr)   with S the synthesis operator of SH_to_spat

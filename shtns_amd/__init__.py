@@ -31,6 +31,8 @@ SHT_SCALAR_ONLY = 256 * 16
 SHT_LOAD_SAVE_CFG = 256 * 64
 SHT_ALLOW_GPU = 256 * 128
 SHT_ALLOW_PADDING = 256 * 256
+# contraction precision of the GPU Legendre step (shtns_hip.h)
+SHT_PREC_AUTO, SHT_PREC_F64, SHT_PREC_F32, SHT_PREC_SPLIT = -1, 0, 1, 2
 
 __all__ = ["sht", "nlm_calc", "build_info"]
 
@@ -254,35 +256,46 @@ class sht:
         s = torch.cuda.current_stream(t.device).cuda_stream
         _lib.load().cushtns_set_streams(self._cfg, ctypes.c_void_p(s), ctypes.c_void_p(s))
 
-    def _torch_spec(self, q):
+    def _torch_spec(self, q, allow_f32=False):
         import torch
         if not q.is_cuda:
             raise RuntimeError("torch tensors must be on the GPU")
-        if q.dtype != torch.complex128:
-            raise RuntimeError("spectral tensor should be torch.complex128")
+        if q.dtype != torch.complex128 and not (allow_f32 and q.dtype == torch.complex64):
+            raise RuntimeError("spectral tensor should be torch.complex128"
+                               + (" or torch.complex64" if allow_f32 else ""))
         self._check_spec(q)
         return q.contiguous()
 
-    def _torch_spat(self, v):
+    def _torch_spat(self, v, allow_f32=False):
         import torch
         if not v.is_cuda:
             raise RuntimeError("torch tensors must be on the GPU")
-        if v.dtype != torch.float64:
-            raise RuntimeError("spatial tensor should be torch.float64")
+        if v.dtype != torch.float64 and not (allow_f32 and v.dtype == torch.float32):
+            raise RuntimeError("spatial tensor should be torch.float64" + (" or torch.float32" if allow_f32 else ""))
         self._check_spat(v)
         return v.contiguous()
 
+    def set_precision(self, prec):
+        """Precision of the Legendre contraction on the GPU: SHT_PREC_F64, SHT_PREC_F32,
+        SHT_PREC_SPLIT or SHT_PREC_AUTO (FP64 for double data, FP32 for float data)."""
+        if _lib.load().cushtns_set_precision(self._cfg, int(prec)) != 0:
+            _lib.check_error("set_precision")
+
     def _synth_torch(self, arg, ltr):
         import torch
-        q = [self._torch_spec(a) for a in arg]
+        q = [self._torch_spec(a, allow_f32=(len(arg) == 1)) for a in arg]
+        f32 = q[0].dtype == torch.complex64
         batch = tuple(q[0].shape[:-1])
         nf = int(np.prod(batch)) if batch else 1
         self._stream(q[0])
         lib = _lib.load()
         lt = self.lmax if ltr < 0 else ltr
-        out = [torch.empty(batch + tuple(self.spat_shape), dtype=torch.float64, device=q[0].device) for _ in q]
+        rdt = torch.float32 if f32 else torch.float64
+        out = [torch.empty(batch + tuple(self.spat_shape), dtype=rdt, device=q[0].device) for _ in q]
         if nf > 0:
-            if len(q) == 1:
+            if f32:
+                lib.cu_SH_to_spat_float_batch(self._cfg, _ptr(q[0]), _ptr(out[0]), lt, nf, self.nlm, self.nspat)
+            elif len(q) == 1:
                 lib.cu_SH_to_spat_batch(self._cfg, _ptr(q[0]), _ptr(out[0]), lt, nf, self.nlm, self.nspat)
             else:
                 lib.cu_SHsphtor_to_spat_batch(self._cfg, _ptr(q[0]), _ptr(q[1]), _ptr(out[0]), _ptr(out[1]), lt, nf,
@@ -292,15 +305,19 @@ class sht:
 
     def _analys_torch(self, arg, ltr):
         import torch
-        v = [self._torch_spat(a) for a in arg]
+        v = [self._torch_spat(a, allow_f32=(len(arg) == 1)) for a in arg]
+        f32 = v[0].dtype == torch.float32
         batch = tuple(v[0].shape[:-2])
         nf = int(np.prod(batch)) if batch else 1
         self._stream(v[0])
         lib = _lib.load()
         lt = self.lmax if ltr < 0 else ltr
-        out = [torch.empty(batch + (self.nlm,), dtype=torch.complex128, device=v[0].device) for _ in v]
+        cdt = torch.complex64 if f32 else torch.complex128
+        out = [torch.empty(batch + (self.nlm,), dtype=cdt, device=v[0].device) for _ in v]
         if nf > 0:
-            if len(v) == 1:
+            if f32:
+                lib.cu_spat_to_SH_float_batch(self._cfg, _ptr(v[0]), _ptr(out[0]), lt, nf, self.nspat, self.nlm)
+            elif len(v) == 1:
                 lib.cu_spat_to_SH_batch(self._cfg, _ptr(v[0]), _ptr(out[0]), lt, nf, self.nspat, self.nlm)
             else:
                 lib.cu_spat_to_SHsphtor_batch(self._cfg, _ptr(v[0]), _ptr(v[1]), _ptr(out[0]), _ptr(out[1]), lt, nf,

@@ -24,6 +24,8 @@ EXPORTED_SYMBOLS = [
     "cu_SH_to_spat_batch", "cu_spat_to_SH_batch", "cu_SHsphtor_to_spat_batch",
     "cu_spat_to_SHsphtor_batch", "cu_SH_to_spat_adjoint_batch", "cu_spat_to_SH_adjoint_batch",
     "shtns_profile", "shtns_profile_get",
+    "cu_spat_to_SH_float", "cu_SH_to_spat_float", "cu_SH_to_spat_float_batch", "cu_spat_to_SH_float_batch",
+    "cushtns_set_precision",
 ]
 
 # public prefix of struct shtns_info (shtns.h:79-94)
@@ -113,6 +115,11 @@ def load():
         "cu_spat_to_SH_adjoint_batch": (None, [CFG, P, P, I, I, L, L]),
         "shtns_profile": (None, [CFG, I]),
         "shtns_profile_get": (I, [CFG, P, P]),
+        "cu_SH_to_spat_float": (None, [CFG, P, P, I]),
+        "cu_spat_to_SH_float": (None, [CFG, P, P, I]),
+        "cu_SH_to_spat_float_batch": (None, [CFG, P, P, I, I, L, L]),
+        "cu_spat_to_SH_float_batch": (None, [CFG, P, P, I, I, L, L]),
+        "cushtns_set_precision": (I, [CFG, I]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,411 @@
+// sht_fft.hip -- longitude FFTs of the transforms (SHTns does these with FFTW/cuFFT:
+// sht_init.c:353-518 plans nlat c2c DFTs of nphi/2 points with two real rows packed as one
+// complex row; sht_gpu.cu:279-303 uses cuFFT Z2Z the same way).
+//
+// Each latitude row of nphi real values is transformed as one complex sequence of
+// M = nphi/2 points z_n = V[2n] + i V[2n+1] (analysis) or produced from one (synthesis).
+// A sequence lives in the registers of LQ lanes, R = M/LQ points per lane (R = 24 when 3
+// divides M, else 16), lane l holding points l + LQ*j, j < R.  The transform is a Stockham
+// (self-sorting, decimation in time) sequence of radix-P passes; the butterflies of a pass
+// whose inputs are a lane's own points run in registers, and the data moves between lanes
+// through LDS only between passes (M = 1536: radices 24, 8, 8 -> two LDS exchanges).  The
+// first pass reads HBM directly and the last pass leaves its outputs in the lane that
+// stores them, so nothing else is staged.
+#include <utility>
+
+#include "sht_internal.h"
+
+namespace shtns_amd {
+namespace {
+
+// ---- compile-time helpers ---------------------------------------------------------------
+template <class F, int... I>
+__device__ __forceinline__ void sfor_(F &&f, std::integer_sequence<int, I...>) {
+    (f(std::integral_constant<int, I>{}), ...);
+}
+// calls f(integral_constant<int, i>) for i = 0 .. N-1
+template <int N, class F>
+__device__ __forceinline__ void sfor(F &&f) {
+    sfor_(f, std::make_integer_sequence<int, N>{});
+}
+
+// cos(2 pi j / 96)
+constexpr double cos96(int j) {
+    constexpr double c[25] = {1.0,
+                              0.997858923238603506738,
+                              0.991444861373810411145,
+                              0.980785280403230449126,
+                              0.96592582628906828675,
+                              0.946930129495105664256,
+                              0.923879532511286756128,
+                              0.896872741532688303894,
+                              0.866025403784438646764,
+                              0.831469612302545237079,
+                              0.79335334029123516458,
+                              0.751839807478977396408,
+                              0.707106781186547524401,
+                              0.659345815100068868425,
+                              0.608761429008720639416,
+                              0.555570233019602224743,
+                              0.5,
+                              0.442288690219001281995,
+                              0.382683432365089771728,
+                              0.321439465303161580701,
+                              0.258819045102520762349,
+                              0.195090322016128267848,
+                              0.130526192220051591548,
+                              0.0654031292301430668153,
+                              0.0};
+    j = ((j % 96) + 96) % 96;
+    return j <= 24 ? c[j] : j <= 48 ? -c[48 - j] : j <= 72 ? -c[j - 48] : c[96 - j];
+}
+constexpr double sin96(int j) { return cos96(j - 24); }
+
+// a * exp(SGN * 2 pi i J / 96), quarter turns without multiplies
+template <int J, int SGN>
+__device__ __forceinline__ double2 rot96(double2 a) {
+    constexpr int j = ((J % 96) + 96) % 96;
+    if constexpr (j == 0) {
+        return a;
+    } else if constexpr (j == 48) {
+        return make_double2(-a.x, -a.y);
+    } else if constexpr ((j == 24 && SGN > 0) || (j == 72 && SGN < 0)) {
+        return make_double2(-a.y, a.x);   // * i
+    } else if constexpr (j == 24 || j == 72) {
+        return make_double2(a.y, -a.x);   // * -i
+    } else {
+        constexpr double c = cos96(j), s = SGN * sin96(j);
+        return make_double2(a.x * c - a.y * s, a.x * s + a.y * c);
+    }
+}
+
+__device__ __forceinline__ double2 c_add(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 c_sub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 c_mul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 c_conj(double2 a) { return make_double2(a.x, -a.y); }
+
+// In-register DFT of P points: y_k = sum_n x_n exp(SGN 2 pi i n k / P), P | 96.
+// Composite P = p1 * p2 splits as n = p2 n1 + n2, k = k1 + p1 k2.
+template <int P, int SGN>
+__device__ __forceinline__ void dft(double2 *x) {
+    if constexpr (P == 1) {
+    } else if constexpr (P == 2) {
+        const double2 a = x[0], b = x[1];
+        x[0] = c_add(a, b);
+        x[1] = c_sub(a, b);
+    } else if constexpr (P == 3) {
+        constexpr double h = 0.86602540378443864676 * SGN;   // sin(2 pi / 3)
+        const double2 s = c_add(x[1], x[2]), d = c_sub(x[1], x[2]);
+        const double2 t = make_double2(x[0].x - 0.5 * s.x, x[0].y - 0.5 * s.y);
+        const double2 u = make_double2(-h * d.y, h * d.x);
+        x[0] = c_add(x[0], s);
+        x[1] = c_add(t, u);
+        x[2] = c_sub(t, u);
+    } else if constexpr (P == 4) {
+        const double2 a = c_add(x[0], x[2]), b = c_sub(x[0], x[2]);
+        const double2 c = c_add(x[1], x[3]), d = rot96<24, SGN>(c_sub(x[1], x[3]));
+        x[0] = c_add(a, c);
+        x[2] = c_sub(a, c);
+        x[1] = c_add(b, d);
+        x[3] = c_sub(b, d);
+    } else {
+        static_assert(96 % P == 0, "radix must divide 96");
+        constexpr int p1 = (P % 4 == 0) ? 4 : (P % 3 == 0 ? 3 : 2);
+        constexpr int p2 = P / p1;
+        double2 t[P];
+        sfor<p2>([&](auto n2c) {
+            constexpr int n2 = decltype(n2c)::value;
+            double2 u[p1];
+            sfor<p1>([&](auto n1c) { u[decltype(n1c)::value] = x[p2 * decltype(n1c)::value + n2]; });
+            dft<p1, SGN>(u);
+            sfor<p1>([&](auto k1c) {
+                constexpr int k1 = decltype(k1c)::value;
+                t[n2 * p1 + k1] = rot96<(96 / P) * n2 * k1, SGN>(u[k1]);
+            });
+        });
+        sfor<p1>([&](auto k1c) {
+            constexpr int k1 = decltype(k1c)::value;
+            double2 u[p2];
+            sfor<p2>([&](auto n2c) { u[decltype(n2c)::value] = t[decltype(n2c)::value * p1 + k1]; });
+            dft<p2, SGN>(u);
+            sfor<p2>([&](auto k2c) { x[k1 + p1 * decltype(k2c)::value] = u[decltype(k2c)::value]; });
+        });
+    }
+}
+
+// ---- plan ---------------------------------------------------------------------------------
+constexpr int wave_R(int M) { return (M % 3 == 0) ? 24 : 16; }
+constexpr bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+constexpr bool wave_fft_ok(int M) { return M % wave_R(M) == 0 && is_pow2(M / wave_R(M)) && M / wave_R(M) <= 256; }
+
+struct WPlan {
+    int n;
+    int r[8];
+};
+// radices for M points with R points per lane: each radix divides R (so a lane's butterflies
+// of a pass read only its own points) and the largest one is taken first
+constexpr WPlan wave_plan(int M, int R) {
+    WPlan p{0, {}};
+    constexpr int cand[8] = {24, 16, 12, 8, 6, 4, 3, 2};
+    int rem = M;
+    while (rem > 1 && p.n < 8) {
+        int pick = 0;
+        for (int c : cand)
+            if (R % c == 0 && rem % c == 0) { pick = c; break; }
+        if (!pick) return WPlan{-1, {}};
+        p.r[p.n++] = pick;
+        rem /= pick;
+    }
+    return p;
+}
+constexpr int plan_p(int M, int R, int S) {   // product of the radices of passes < S
+    WPlan w = wave_plan(M, R);
+    int p = 1;
+    for (int s = 0; s < S; ++s) p *= w.r[s];
+    return p;
+}
+
+// LDS slot of point j of a sequence: one spare slot every 16 points against bank conflicts
+__device__ __forceinline__ int lds_slot(int j) { return j + (j >> 4); }
+constexpr int lds_seq_stride(int M) { return M + M / 16 + 1; }
+
+template <int M>
+struct WaveGeom {
+    static constexpr int R = wave_R(M);
+    static constexpr int LQ = M / R;                   // lanes per sequence
+    static constexpr int NT = LQ >= 64 ? LQ : 64;      // threads per block
+    static constexpr int SPB = NT / LQ;                // sequences (latitudes) per block
+    static constexpr WPlan PL = wave_plan(M, R);
+    static constexpr size_t lds_bytes = (size_t)SPB * lds_seq_stride(M) * sizeof(double2);
+};
+
+// Passes S.. of the Stockham FFT on the lane's points v[j] = x[l + LQ j].  Pass S (radix P,
+// p = product of the earlier radices) runs butterflies i = l + LQ b, b < R/P, on inputs
+// x[i + r M/P] = v[b + r R/P], multiplies input r by exp(SGN 2 pi i r k/(pP)), k = i mod p,
+// and sends output r to position (i - k) P + k + r p.  On the last pass that position is the
+// lane's own v[b + r R/P], so it stays in registers.
+template <int M, int SGN, int S>
+__device__ __forceinline__ void wave_passes(double2 (&v)[WaveGeom<M>::R], double2 *seq, int l,
+                                            const double2 *__restrict__ roots) {
+    using G = WaveGeom<M>;
+    constexpr int R = G::R, LQ = G::LQ;
+    if constexpr (S < G::PL.n) {
+        constexpr int P = G::PL.r[S];
+        constexpr int p = plan_p(M, R, S);
+        constexpr int NB = R / P;
+        constexpr bool last = (S + 1 == G::PL.n);
+        constexpr int tw_step = 2 * M / (p * P);   // roots hold exp(2 pi i e / (2M))
+        sfor<NB>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            const int i = l + LQ * b;
+            const int k = (p == 1) ? 0 : i % p;
+            double2 u[P];
+            sfor<P>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                u[r] = v[b + r * NB];
+                if constexpr (p > 1 && r > 0) {
+                    double2 w = roots[r * k * tw_step];
+                    if constexpr (SGN < 0) w.y = -w.y;
+                    u[r] = c_mul(u[r], w);
+                }
+            });
+            dft<P, SGN>(u);
+            if constexpr (last) {
+                sfor<P>([&](auto rc) { v[b + decltype(rc)::value * NB] = u[decltype(rc)::value]; });
+            } else {
+                const int base = (i - k) * P + k;
+                sfor<P>([&](auto rc) { seq[lds_slot(base + decltype(rc)::value * p)] = u[decltype(rc)::value]; });
+            }
+        });
+        if constexpr (!last) {
+            __syncthreads();
+            sfor<R>([&](auto jc) { v[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+            __syncthreads();
+            wave_passes<M, SGN, S + 1>(v, seq, l, roots);
+        }
+    }
+}
+
+// Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous range of latitude
+// groups so that the blocks running together on one XCD touch neighbouring latitudes (which
+// share cache lines of that XCD's L2 in the theta-contiguous layout).
+__device__ __forceinline__ int xcd_group_w(int bid, int G) {
+    if (G % 8) return bid;
+    return (bid % 8) * (G / 8) + bid / 8;
+}
+
+// Synthesis: Fourier coefficients F[f][k][lat] (k <= imlim, zero above) -> real rows
+// V[f] at (2n)*sp + lat*stt, (2n+1)*sp + lat*stt.  The packed sequence is
+// Z_k = A_k + i w_k B_k with A_k = X_k + conj X_{M-k}, B_k = X_k - conj X_{M-k}, w_k = e^{i pi k/M},
+// and z_n = sum_k Z_k e^{2 pi i n k/M} = V[2n] + i V[2n+1].
+template <int M, class TV>
+__global__ __launch_bounds__(WaveGeom<M>::NT) void fft_synth_wave(const double2 *__restrict__ F, long four_dist,
+                                                                  TV *__restrict__ V, long spat_dist,
+                                                                  const double2 *__restrict__ roots, int nlat,
+                                                                  int imlim, long sp, long stt, int lat_base,
+                                                                  int lat_end) {
+    using G = WaveGeom<M>;
+    constexpr int R = G::R, LQ = G::LQ;
+    extern __shared__ double2 lds[];
+    const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
+    const int lat = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * G::SPB + sq;
+    const bool live = lat < lat_end;
+    const double2 *Ff = F + (size_t)blockIdx.y * four_dist + lat;
+    double2 *seq = lds + sq * lds_seq_stride(M);
+    const double2 wl = roots[l];   // w_k = w_l * exp(i pi j / R) for k = l + LQ j
+    double2 v[R];
+    sfor<R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int k = l + LQ * j, km = M - k;
+        double2 xk = make_double2(0.0, 0.0), xm = make_double2(0.0, 0.0);
+        if (live && k <= imlim) xk = Ff[(size_t)k * nlat];
+        if (live && k > 0 && km <= imlim) xm = Ff[(size_t)km * nlat];
+        const double2 A = c_add(xk, c_conj(xm)), B = c_sub(xk, c_conj(xm));
+        const double2 w = rot96<(96 / (2 * R)) * j, 1>(wl);
+        const double2 wB = c_mul(w, B);
+        v[j] = make_double2(A.x - wB.y, A.y + wB.x);
+    });
+    wave_passes<M, 1, 0>(v, seq, l, roots);
+    if (live) {
+        TV *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)lat * stt;
+        sfor<R>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const long n = l + LQ * j;
+            Vf[(2 * n) * sp] = (TV)v[j].x;
+            Vf[(2 * n + 1) * sp] = (TV)v[j].y;
+        });
+    }
+}
+
+// Analysis: real rows -> F[f][k][lat], k <= mmax (zero above imlim).  With Z_k the forward
+// DFT (sign -1) of z_n = V[2n] + i V[2n+1]: X_k = A + conj(w_k) B, A = (Z_k + conj Z_{M-k})/2,
+// B = (Z_k - conj Z_{M-k})/(2i); Im X_0 = 0.
+template <int M, class TV>
+__global__ __launch_bounds__(WaveGeom<M>::NT) void fft_anal_wave(const TV *__restrict__ V, long spat_dist,
+                                                                 double2 *__restrict__ F, long four_dist,
+                                                                 const double2 *__restrict__ roots, int nlat,
+                                                                 int mmax, int imlim, long sp, long stt,
+                                                                 int lat_base, int lat_end) {
+    using G = WaveGeom<M>;
+    constexpr int R = G::R, LQ = G::LQ;
+    extern __shared__ double2 lds[];
+    const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
+    const int lat = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * G::SPB + sq;
+    const bool live = lat < lat_end;
+    double2 *seq = lds + sq * lds_seq_stride(M);
+    double2 v[R];
+    {
+        const TV *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(live ? lat : 0) * stt;
+        sfor<R>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const long n = l + LQ * j;
+            v[j] = live ? make_double2((double)Vf[(2 * n) * sp], (double)Vf[(2 * n + 1) * sp]) : make_double2(0.0, 0.0);
+        });
+    }
+    const double2 wl = roots[l];
+    wave_passes<M, -1, 0>(v, seq, l, roots);
+    // Z_{M-k} sits in another lane: exchange through LDS
+    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value]; });
+    __syncthreads();
+    if (live) {
+        double2 *Ff = F + (size_t)blockIdx.y * four_dist + lat;
+        sfor<R>([&](auto jc) {
+            constexpr int j = decltype(jc)::value;
+            const int k = l + LQ * j;
+            if (k <= mmax) {
+                double2 X = make_double2(0.0, 0.0);
+                if (k <= imlim) {
+                    const double2 Zk = v[j];
+                    const double2 Zm = seq[lds_slot(k == 0 ? 0 : M - k)];
+                    const double2 A = make_double2(0.5 * (Zk.x + Zm.x), 0.5 * (Zk.y - Zm.y));
+                    const double2 D = make_double2(Zk.x - Zm.x, Zk.y + Zm.y);
+                    const double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
+                    const double2 w = c_conj(rot96<(96 / (2 * R)) * j, 1>(wl));
+                    X = c_add(A, c_mul(w, B));
+                    if (k == 0) X.y = 0.0;
+                }
+                Ff[(size_t)k * nlat] = X;
+            }
+        });
+    }
+}
+
+template <class K>
+void lds_limit(K kern, size_t lds) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+}
+
+}  // namespace
+
+#define WAVE_FFT_SIZES(X) X(16) X(24) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) \
+    X(1024) X(1536) X(2048) X(3072) X(4096) X(6144)
+
+bool fft_wave_supported(int M) {
+#define OK_CASE(MM) if (M == MM) return true;
+    WAVE_FFT_SIZES(OK_CASE)
+#undef OK_CASE
+    return false;
+}
+
+namespace {
+template <class TV>
+bool synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F, long four_dist,
+                TV *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+#define SYN_CASE(MM)                                                                                          \
+    if (M == MM) {                                                                                            \
+        static_assert(wave_fft_ok(MM) && WaveGeom<MM>::PL.n > 0, "unsupported wave FFT size");                \
+        using G = WaveGeom<MM>;                                                                               \
+        lds_limit(fft_synth_wave<MM, TV>, G::lds_bytes);                                                      \
+        const int nb = (lat_end - lat_base + G::SPB - 1) / G::SPB;                                           \
+        hipLaunchKernelGGL((fft_synth_wave<MM, TV>), dim3(nb, nfields), dim3(G::NT), G::lds_bytes, s, F,      \
+                           four_dist, V, spat_dist, t->roots, nlat, imlim, sp, st, lat_base, lat_end);        \
+        return true;                                                                                          \
+    }
+    WAVE_FFT_SIZES(SYN_CASE)
+#undef SYN_CASE
+    return false;
+}
+
+template <class TV>
+bool anal_wave(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const TV *V, long spat_dist,
+               double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+#define ANA_CASE(MM)                                                                                          \
+    if (M == MM) {                                                                                            \
+        using G = WaveGeom<MM>;                                                                               \
+        lds_limit(fft_anal_wave<MM, TV>, G::lds_bytes);                                                       \
+        const int nb = (lat_end - lat_base + G::SPB - 1) / G::SPB;                                           \
+        hipLaunchKernelGGL((fft_anal_wave<MM, TV>), dim3(nb, nfields), dim3(G::NT), G::lds_bytes, s, V,       \
+                           spat_dist, F, four_dist, t->roots, nlat, mmax, imlim, sp, st, lat_base, lat_end);  \
+        return true;                                                                                          \
+    }
+    WAVE_FFT_SIZES(ANA_CASE)
+#undef ANA_CASE
+    return false;
+}
+}  // namespace
+
+bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
+                           long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
+                           int lat_end) {
+    return synth_wave(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
+}
+bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
+                           long four_dist, float *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
+                           int lat_end) {
+    return synth_wave(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
+}
+bool launch_fft_anal_wave(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
+                          long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
+                          int lat_end) {
+    return anal_wave(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end);
+}
+bool launch_fft_anal_wave(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const float *V,
+                          long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
+                          int lat_end) {
+    return anal_wave(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end);
+}
+
+}  // namespace shtns_amd

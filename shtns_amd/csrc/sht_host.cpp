@@ -216,6 +216,9 @@ static void free_device(shtns_info *c) {
             if (p) (void)hipFree(p);
     }
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->xscratch) (void)hipFree(c->xscratch);
+    c->xscratch = nullptr;
+    c->xscratch_bytes = 0;
     if (c->own_streams) {
         if (c->comp_stream) (void)hipStreamDestroy(c->comp_stream);
     }
@@ -609,6 +612,8 @@ extern "C" shtns_cfg cushtns_clone(shtns_cfg c, hipStream_t compute_stream, hipS
     memcpy(k->ct, c->ct, sizeof(double) * 2 * c->nlat);
     k->st = k->ct + c->nlat;
     k->scratch = nullptr;
+    k->xscratch = nullptr;
+    k->xscratch_bytes = 0;
     k->scratch_bytes = 0;
     k->own_streams = false;
     k->comp_stream = compute_stream;
@@ -759,12 +764,35 @@ extern "C" int shtns_profile_get(shtns_cfg c, double *ms, int *count) {
     return PK_N;
 }
 
+// contraction precision for double data / float data (cushtns_set_precision)
+static int prec_f64(shtns_cfg c) { return c->prec >= 0 ? c->prec : kPrecF64; }
+static int prec_f32(shtns_cfg c) { return c->prec >= 0 ? c->prec : kPrecF32; }
+
+static void *xscratch(shtns_cfg c, size_t bytes) {
+    if (bytes <= c->xscratch_bytes) return c->xscratch;
+    if (c->xscratch) {
+        (void)hipStreamSynchronize(c->comp_stream);
+        (void)hipFree(c->xscratch);
+    }
+    c->xscratch = nullptr;
+    c->xscratch_bytes = 0;
+    if (hipMalloc(&c->xscratch, bytes) != hipSuccess) {
+        set_error(2, "staging allocation of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    c->xscratch_bytes = bytes;
+    return c->xscratch;
+}
+
 // synthesis core: Q (nfields, spec_dist) -> V (nfields, spat_dist)
 // The Legendre kernel runs in chunks of latitude tiles on the compute stream; the FFT of a
 // chunk's latitudes (north and south rows) runs on the second stream as soon as the chunk
 // is done, overlapping the next chunk's Legendre work (matrix cores vs LDS/HBM).
+// V is double (vf == nullptr) or, for the FP32-data entry points, float: vf != nullptr, with
+// vstage an FP64 staging area of nfields * nspat doubles when the register FFT cannot take
+// the length (then the FFT writes vstage and a conversion kernel writes vf).
 static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
-                       double mpos_scale, int lat_weight) {
+                       double mpos_scale, int lat_weight, int prec, float *vf = nullptr, double *vstage = nullptr) {
     if (nfields <= 0) return;
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
@@ -778,13 +806,14 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     LegendreArgs a = leg_args(c, ltr, imlim, nfields);
     a.mpos_scale = mpos_scale;
     a.lat_weight = lat_weight;
+    a.prec = prec;
     long sp, st;
     spat_strides(c, sp, st);
     const hipStream_t S = c->comp_stream;
     const int ntiles = (c->nlat_2 + 63) / 64;
     const int chunk_tiles = 4;
     const int nchunks = (ntiles + chunk_tiles - 1) / chunk_tiles;
-    hipStream_t A = (overlap_enabled() && nchunks > 1) ? aux_stream(c) : nullptr;
+    hipStream_t A = (overlap_enabled() && nchunks > 1 && !vf) ? aux_stream(c) : nullptr;
     {
         ProfScope ps(c, PK_LEG_SYNTH);
         launch_premix(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, qmix, c->nsteps_total, S);
@@ -795,7 +824,16 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
             launch_legendre_synth_tiles(a, c->lmax, F, four_dist, qmix, c->nsteps_total, 0, ntiles, S);
         }
         ProfScope ps(c, PK_FFT_SYNTH);
-        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S);
+        if (!vf) {
+            launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S);
+        } else if (!vstage) {
+            launch_fft_synth_wave(&c->d, c->nlat, c->nphi / 2, imlim, nfields, F, four_dist, vf, spat_dist, sp, st, S,
+                                  0, c->nlat);
+        } else {
+            launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, vstage, c->nspat, sp, st,
+                             S);
+            launch_convert(vstage, c->nspat, vf, spat_dist, c->nspat, nfields, S);
+        }
         HIP_OK(hipGetLastError());
         return;
     }
@@ -823,8 +861,11 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
 
 // analysis core: V -> Q.  With 16+ fields the FFTs of 8-field chunks run on the second
 // stream ahead of the Legendre analysis of the previous chunk on the compute stream.
+// V is double (vf == nullptr) or float (vf != nullptr, vstage as in synth_core: when set,
+// vf is first converted into it).
 static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
-                      double mpos_scale, int lat_weight) {
+                      double mpos_scale, int lat_weight, int prec, const float *vf = nullptr,
+                      double *vstage = nullptr) {
     if (nfields <= 0) return;
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
@@ -836,15 +877,26 @@ static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfield
     const hipStream_t S = c->comp_stream;
     const int chunk = 8;
     const int nchunks = (nfields + chunk - 1) / chunk;
-    hipStream_t A = (overlap_enabled() && nchunks > 1) ? aux_stream(c) : nullptr;
+    hipStream_t A = (overlap_enabled() && nchunks > 1 && !vf) ? aux_stream(c) : nullptr;
     if (!A) {
         {
             ProfScope ps(c, PK_FFT_ANAL);
-            launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, S);
+            if (!vf) {
+                launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st,
+                                S);
+            } else if (!vstage) {
+                launch_fft_anal_wave(&c->d, c->nlat, c->nphi / 2, c->mmax, imlim, nfields, vf, spat_dist, F,
+                                     four_dist, sp, st, S, 0, c->nlat);
+            } else {
+                launch_convert(vf, spat_dist, vstage, c->nspat, c->nspat, nfields, S);
+                launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, vstage, c->nspat, F, four_dist, sp,
+                                st, S);
+            }
         }
         LegendreArgs a = leg_args(c, ltr, imlim, nfields);
         a.mpos_scale = mpos_scale;
         a.lat_weight = lat_weight;
+        a.prec = prec;
         ProfScope ps(c, PK_LEG_ANAL);
         launch_legendre_anal(a, F, four_dist, reinterpret_cast<double2 *>(Q), spec_dist, c->lmax, S);
         HIP_OK(hipGetLastError());
@@ -866,6 +918,7 @@ static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfield
         LegendreArgs a = leg_args(c, ltr, imlim, nf);
         a.mpos_scale = mpos_scale;
         a.lat_weight = lat_weight;
+        a.prec = prec;
         ProfScope ps(c, PK_LEG_ANAL);
         launch_legendre_anal(a, F + (size_t)f0 * four_dist, four_dist, reinterpret_cast<double2 *>(Q) + (size_t)f0 * spec_dist,
                              spec_dist, c->lmax, S);
@@ -892,6 +945,7 @@ static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, d
                         vw_dist, c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);
     LegendreArgs a = leg_args(c, ltr + 1, imlim, 2 * nfields);
     a.vector_mode = 1;
+    a.prec = prec_f64(c);
     launch_legendre_synth(a, VW, vw_dist, c->lmax + 1, F, four_dist, qmix, c->nsteps_total, c->comp_stream);
     long sp, st;
     spat_strides(c, sp, st);
@@ -924,6 +978,7 @@ static void vanal_core(shtns_cfg c, const double *Vt, const double *Vp, cplx *S,
     LegendreArgs a = leg_args(c, ltr + 1, imlim, 2 * nfields);
     a.vector_mode = 1;
     a.lat_weight = 1;
+    a.prec = prec_f64(c);
     launch_legendre_anal(a, F, four_dist, VW, vw_dist, c->lmax + 1, c->comp_stream);
     launch_vw_to_sphtor(VW, vw_dist, reinterpret_cast<double2 *>(S), reinterpret_cast<double2 *>(T), spec_dist,
                         c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);
@@ -934,18 +989,65 @@ static void vanal_core(shtns_cfg c, const double *Vt, const double *Vp, cplx *S,
 extern "C" void cu_SH_to_spat_batch(shtns_cfg c, cplx *Q, double *V, int ltr, int nf, long spec_dist, long spat_dist) {
     clear_error();
     if (!ready(c, "cu_SH_to_spat")) return;
-    synth_core(c, Q, V, ltr, nf, spec_dist, spat_dist, 1.0, 0);
+    synth_core(c, Q, V, ltr, nf, spec_dist, spat_dist, 1.0, 0, prec_f64(c));
 }
 extern "C" void cu_spat_to_SH_batch(shtns_cfg c, double *V, cplx *Q, int ltr, int nf, long spat_dist, long spec_dist) {
     clear_error();
     if (!ready(c, "cu_spat_to_SH")) return;
-    anal_core(c, V, Q, ltr, nf, spat_dist, spec_dist, 1.0, 1);
+    anal_core(c, V, Q, ltr, nf, spat_dist, spec_dist, 1.0, 1, prec_f64(c));
 }
 extern "C" void cu_SH_to_spat(shtns_cfg c, cplx *Q, double *V, int ltr) {
     cu_SH_to_spat_batch(c, Q, V, ltr, 1, c->nlm, c->nspat);
 }
 extern "C" void cu_spat_to_SH(shtns_cfg c, double *V, cplx *Q, int ltr) {
     cu_spat_to_SH_batch(c, V, Q, ltr, 1, c->nspat, c->nlm);
+}
+extern "C" void cu_SH_to_spat_float_batch(shtns_cfg c, cplx_f *Q, float *V, int ltr, int nf, long spec_dist,
+                                          long spat_dist) {
+    clear_error();
+    if (!ready(c, "cu_SH_to_spat_float")) return;
+    if (nf <= 0) return;
+    const bool stage = !fft_wave_supported(c->nphi / 2);
+    const size_t qb = (size_t)nf * c->nlm * sizeof(double2);
+    char *base = (char *)xscratch(c, qb + (stage ? (size_t)nf * c->nspat * sizeof(double) : 0));
+    if (!base) return;
+    double2 *qd = (double2 *)base;
+    launch_convert(reinterpret_cast<const float *>(Q), 2 * spec_dist, reinterpret_cast<double *>(qd), 2L * c->nlm,
+                   2L * c->nlm, nf, c->comp_stream);
+    synth_core(c, reinterpret_cast<const cplx *>(qd), nullptr, ltr, nf, c->nlm, spat_dist, 1.0, 0, prec_f32(c), V,
+               stage ? (double *)(base + qb) : nullptr);
+}
+extern "C" void cu_spat_to_SH_float_batch(shtns_cfg c, float *V, cplx_f *Q, int ltr, int nf, long spat_dist,
+                                          long spec_dist) {
+    clear_error();
+    if (!ready(c, "cu_spat_to_SH_float")) return;
+    if (nf <= 0) return;
+    const bool stage = !fft_wave_supported(c->nphi / 2);
+    const size_t qb = (size_t)nf * c->nlm * sizeof(double2);
+    char *base = (char *)xscratch(c, qb + (stage ? (size_t)nf * c->nspat * sizeof(double) : 0));
+    if (!base) return;
+    double2 *qd = (double2 *)base;
+    anal_core(c, nullptr, reinterpret_cast<cplx *>(qd), ltr, nf, spat_dist, c->nlm, 1.0, 1, prec_f32(c), V,
+              stage ? (double *)(base + qb) : nullptr);
+    if (g_last_err) return;
+    launch_convert(reinterpret_cast<const double *>(qd), 2L * c->nlm, reinterpret_cast<float *>(Q), 2 * spec_dist,
+                   2L * c->nlm, nf, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+extern "C" void cu_SH_to_spat_float(shtns_cfg c, cplx_f *Q, float *V, int ltr) {
+    cu_SH_to_spat_float_batch(c, Q, V, ltr, 1, c->nlm, c->nspat);
+}
+extern "C" void cu_spat_to_SH_float(shtns_cfg c, float *V, cplx_f *Q, int ltr) {
+    cu_spat_to_SH_float_batch(c, V, Q, ltr, 1, c->nspat, c->nlm);
+}
+extern "C" int cushtns_set_precision(shtns_cfg c, int prec) {
+    clear_error();
+    if (c == nullptr || prec < SHT_PREC_AUTO || prec > SHT_PREC_SPLIT) {
+        set_error(1, "cushtns_set_precision: unknown precision %d", prec);
+        return -1;
+    }
+    c->prec = prec;
+    return 0;
 }
 extern "C" void cu_SHsphtor_to_spat_batch(shtns_cfg c, cplx *S, cplx *T, double *Vt, double *Vp, int ltr, int nf,
                                           long spec_dist, long spat_dist) {
@@ -969,13 +1071,13 @@ extern "C" void cu_SH_to_spat_adjoint_batch(shtns_cfg c, double *V, cplx *Q, int
                                             long spec_dist) {
     clear_error();
     if (!ready(c, "cu_SH_to_spat_adjoint")) return;
-    anal_core(c, V, Q, ltr, nf, spat_dist, spec_dist, 2.0, 0);
+    anal_core(c, V, Q, ltr, nf, spat_dist, spec_dist, 2.0, 0, prec_f64(c));
 }
 extern "C" void cu_spat_to_SH_adjoint_batch(shtns_cfg c, cplx *Q, double *V, int ltr, int nf, long spec_dist,
                                             long spat_dist) {
     clear_error();
     if (!ready(c, "cu_spat_to_SH_adjoint")) return;
-    synth_core(c, Q, V, ltr, nf, spec_dist, spat_dist, 0.5, 1);
+    synth_core(c, Q, V, ltr, nf, spec_dist, spat_dist, 0.5, 1, prec_f64(c));
 }
 
 // ---- host-pointer API: copy in, run on the GPU, copy out (synchronous) ----
@@ -1009,7 +1111,7 @@ extern "C" void SH_to_spat_l(shtns_cfg c, cplx *Qlm, double *Vr, int ltr) {
     double *dv = (double *)st.dev(vb);
     if (!dq || !dv) { set_error(2, "SH_to_spat: device allocation failed"); return; }
     if (!h2d(dq, Qlm, qb, c->comp_stream)) { set_error(2, "SH_to_spat: copy failed"); return; }
-    synth_core(c, dq, dv, ltr, 1, c->nlm, c->nspat, 1.0, 0);
+    synth_core(c, dq, dv, ltr, 1, c->nlm, c->nspat, 1.0, 0, prec_f64(c));
     if (g_last_err) return;
     if (!d2h(Vr, dv, vb, c->comp_stream)) { set_error(2, "SH_to_spat: copy failed"); return; }
     HIP_OK(hipStreamSynchronize(c->comp_stream));
@@ -1024,7 +1126,7 @@ extern "C" void spat_to_SH_l(shtns_cfg c, double *Vr, cplx *Qlm, int ltr) {
     double *dv = (double *)st.dev(vb);
     if (!dq || !dv) { set_error(2, "spat_to_SH: device allocation failed"); return; }
     if (!h2d(dv, Vr, vb, c->comp_stream)) { set_error(2, "spat_to_SH: copy failed"); return; }
-    anal_core(c, dv, dq, ltr, 1, c->nspat, c->nlm, 1.0, 1);
+    anal_core(c, dv, dq, ltr, 1, c->nspat, c->nlm, 1.0, 1, prec_f64(c));
     if (g_last_err) return;
     if (!d2h(Qlm, dq, qb, c->comp_stream)) { set_error(2, "spat_to_SH: copy failed"); return; }
     HIP_OK(hipStreamSynchronize(c->comp_stream));

@@ -86,6 +86,12 @@ struct shtns_info {
     // optional per-kernel timing with HIP events on the compute stream (shtns_profile)
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> prof_ev[8];
+    // contraction precision chosen with cushtns_set_precision (-1: FP64 for double data,
+    // FP32 for float data)
+    int prec = -1;
+    // staging for the FP32-data entry points (FP64 copies of spectra / spatial fields)
+    void *xscratch = nullptr;
+    size_t xscratch_bytes = 0;
 };
 
 namespace shtns_amd {
@@ -98,7 +104,11 @@ struct LegendreArgs {
     int vector_mode;          // 0: scalar functions, 1: Y/sin(theta) (spin-1 V,W spectra)
     double mpos_scale;        // factor on m>0 orders (adjoints use 2 or 0.5)
     int lat_weight;           // synthesis: multiply output by wq_k; analysis: 0 => unit weights
+    int prec = 0;             // contraction precision: 0 FP64, 1 FP32, 2 FP32 hi/lo split (sht_legendre.h)
 };
+
+// Contraction precisions of the Legendre kernels (SHT_PREC_* in shtns_hip.h)
+constexpr int kPrecF64 = 0, kPrecF32 = 1, kPrecSplit = 2;
 
 // Fourier buffer layout: F[field][im][nlat] complex (north to south).
 // qmix: scratch of synth_qmix_doubles(nfields, nsteps_total) doubles (pre-mixed coefficients)
@@ -116,12 +126,40 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                      const double *V, long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s,
                      int lat_base = 0, int nlat_chunk = -1);
+// Register-resident FFT kernels (sht_fft.hip) for M = nphi/2 in {16, 24} x 2^k up to 6144,
+// latitudes [lat_base, lat_end); return false (nothing launched) for other M.
+bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
+                           long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
+                           int lat_end);
+bool launch_fft_anal_wave(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
+                          long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
+                          int lat_end);
+// the same for FP32 spatial arrays (transform in FP64, values converted at load / store)
+bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
+                           long four_dist, float *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
+                           int lat_end);
+bool launch_fft_anal_wave(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const float *V,
+                          long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
+                          int lat_end);
+bool fft_wave_supported(int M);
+// element-wise FP32 <-> FP64 copies of nfields arrays of n scalars (sht_kernels.hip)
+void launch_convert(const float *src, long src_dist, double *dst, long dst_dist, long n, int nfields, hipStream_t s);
+void launch_convert(const double *src, long src_dist, float *dst, long dst_dist, long n, int nfields, hipStream_t s);
 // the two halves of launch_legendre_synth: coefficient pre-mix, then the Legendre kernel for
 // latitude tiles [tile0, tile0 + ntl) (64 latitude pairs per tile)
 void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop, double *qmix,
                    int nsteps_total, hipStream_t s);
 void launch_legendre_synth_tiles(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
                                  int nsteps_total, int tile0, int ntl, hipStream_t s);
+// per-precision instantiations (sht_legendre_f32.hip, sht_legendre_split.hip)
+void launch_legendre_synth_tiles_f32(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist,
+                                     const double *qmix, int nsteps_total, int tile0, int ntl, hipStream_t s);
+void launch_legendre_synth_tiles_split(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist,
+                                       const double *qmix, int nsteps_total, int tile0, int ntl, hipStream_t s);
+void launch_legendre_anal_f32(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Qlm, long spec_dist,
+                              int spec_ltop, hipStream_t s);
+void launch_legendre_anal_split(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Qlm,
+                                long spec_dist, int spec_ltop, hipStream_t s);
 void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s);
 void launch_sphtor_to_vw(const double2 *S, const double2 *T, long spec_dist, double2 *VW, long vw_dist,
                          int lmax, int mmax, int mres, int ltr, int imlim, int nfields, hipStream_t s);

@@ -1,0 +1,584 @@
+// sht_legendre.h -- Legendre-step kernels of shtns_amd as templates, instantiated per
+// contraction precision in sht_kernels.hip (FP64), sht_legendre_f32.hip (FP32) and
+// sht_legendre_split.hip (FP32 hi/lo split).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
+
+#include "sht_internal.h"
+
+namespace shtns_amd {
+
+typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------------------
+// Helpers shared by the Legendre kernels
+// ---------------------------------------------------------------------------------------
+// Read-only table viewed through the constant address space: loads with wave-uniform
+// indices become scalar (SMEM) loads and stay off the vector-memory counters.
+struct UniformTable2 {
+    const __attribute__((address_space(4))) double *p;
+    __device__ explicit UniformTable2(const double2 *q) : p((const __attribute__((address_space(4))) double *)q) {}
+    __device__ double2 operator[](int i) const { return make_double2(p[2 * i], p[2 * i + 1]); }
+};
+struct OrderInfo {
+    int slot;       // table slot (im, or mmax+1 for the m=1 functions of vector m=0)
+    int mfun;       // order of the generated functions
+    int nsteps;     // recurrence steps to evaluate (degrees mfun .. ltr)
+    int spec_base;  // index of degree 0 of this order in the spectral layout (may be virtual)
+    bool sinv;      // divide functions by sin(theta) (vector V,W spectra, m>0)
+    bool real_only; // m = 0: imaginary parts ignored / zero
+};
+
+__device__ inline OrderInfo order_info(int im, int lmax_spec, int mmax, int mres, int ltr, int vector_mode) {
+    OrderInfo o;
+    const int m = im * mres;
+    o.real_only = (im == 0);
+    if (vector_mode && im == 0) {
+        o.slot = mmax + 1;
+        o.mfun = 1;
+        o.sinv = false;
+    } else {
+        o.slot = im;
+        o.mfun = m;
+        o.sinv = (vector_mode != 0);
+    }
+    o.nsteps = (ltr >= o.mfun) ? (ltr - o.mfun) / 2 + 1 : 0;
+    o.spec_base = (int)(((long)im * (2 * lmax_spec + 2 - (im + 1) * mres)) / 2);
+    return o;
+}
+
+// ---------------------------------------------------------------------------------------
+// Contraction precision PREC (the recurrence, seeds, tables and the final sums stay FP64):
+//   0  FP64 matrix cores: v_mfma_f64_16x16x4f64 on z and the coefficients;
+//   1  FP32 matrix cores: z and the coefficients rounded to FP32, v_mfma_f32_16x16x4f32,
+//      FP32 partial sums over 16 degrees-pairs added into FP64;
+//   2  FP32 split: z = z_hi + z_lo and b = b_hi + b_lo (FP32 pairs) with the three products
+//      z_hi b_hi + z_lo b_hi + z_hi b_lo on the FP32 matrix cores, same FP32 partial sums.
+// A recurrence value sits in one 8-byte LDS slot either as a double or as an FP32 (hi, lo) pair.
+// ---------------------------------------------------------------------------------------
+template <int PREC>
+struct Contract {
+    using acc_t = std::conditional_t<PREC == 0, double4_t, float4_t>;
+    using a_t = std::conditional_t<PREC == 0, double, float2>;    // A operand (recurrence value)
+    using b_t = std::conditional_t<PREC == 0, double, float2>;    // B operand (coefficient or data)
+    __device__ static acc_t zero() {
+        if constexpr (PREC == 0) return (double4_t){0.0, 0.0, 0.0, 0.0};
+        else return (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+    }
+    __device__ static void store_z(double *slot, double z) {
+        if constexpr (PREC == 0) {
+            *slot = z;
+        } else {
+            const float hi = (float)z;
+            *reinterpret_cast<float2 *>(slot) = make_float2(hi, (float)(z - (double)hi));
+        }
+    }
+    __device__ static a_t load_z(const double *slot) {
+        if constexpr (PREC == 0) return *slot;
+        else return *reinterpret_cast<const float2 *>(slot);
+    }
+    __device__ static b_t split(double b) {
+        if constexpr (PREC == 0) {
+            return b;
+        } else {
+            const float hi = (float)b;
+            return make_float2(hi, (float)(b - (double)hi));
+        }
+    }
+    // acc += A * B for one 16x16x4 tile
+    __device__ static acc_t mma(a_t a, b_t b, acc_t acc) {
+        if constexpr (PREC == 0) {
+            return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+        } else if constexpr (PREC == 1) {
+            return __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc, 0, 0, 0);
+        } else {
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, acc, 0, 0, 0);
+            return __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, acc, 0, 0, 0);
+        }
+    }
+    // D layout on gfx950: register j of lane l holds D[row][l % 16] with
+    //   f64 16x16x4: row = 4 j + l / 16;   f32 16x16x4: row = 4 (l / 16) + j.
+    __device__ static int d_row(int j, int lane) {
+        if constexpr (PREC == 0) return 4 * j + (lane >> 4);
+        else return 4 * (lane >> 4) + j;
+    }
+};
+
+// ---------------------------------------------------------------------------------------
+// Synthesis, step 2: F[f][im][k] = sum_l Q_lm P_lm(x_k)   (both hemispheres)
+// One wave = one order im, 4*NCG fields, one latitude tile of 64 pairs (lane = pair); waves
+// are independent (no block barrier).  Software pipeline per 4-step block:
+//   A operands (z of 16 latitudes x 4 steps) read back from the wave's LDS tile,
+//   4 x NCG MFMAs D[16 lat][16 col] += A * B,
+//   then, while the matrix cores work, the recurrence of the next 4 steps is advanced on
+//   the vector ALUs into the other LDS tile and the next B operands are loaded.
+// The accumulators go straight to F: lanes holding E and O of one field are lane^2
+// partners (north = E + x O, south = E - x O).
+// ---------------------------------------------------------------------------------------
+constexpr int kSynthWaves = 4;
+
+template <int NCG, int PREC>
+__global__ __launch_bounds__(256) void legendre_synth_kernel(
+    const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
+    const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
+    const int *__restrict__ i0tab, const double2 *__restrict__ z0tab, const double *__restrict__ qmix,
+    int nsteps_total, int lmax_spec, double2 *__restrict__ F, long four_dist, int mmax, int mres, int nlat,
+    int nlat_2, int ltr, int imlim, int nfields, int vector_mode, int lat_weight, int tile0, int ntl) {
+    constexpr int NCOL = 16 * NCG;
+    __shared__ double ytile[kSynthWaves][2][4][64];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    // heavy orders first: block x -> (order, tile group); all tile groups of an order adjacent
+    const int ntg = (ntl + kSynthWaves - 1) / kSynthWaves;   // tile groups of this launch
+    const int im = blockIdx.x / ntg;
+    const int tile = tile0 + (blockIdx.x - im * ntg) * kSynthWaves + wave;
+    const int g = blockIdx.y;
+    const int fbase = g * 4 * NCG;
+    if (tile >= tile0 + ntl || tile * 64 >= nlat_2) return;   // no block-level synchronization in this kernel
+    const int k = tile * 64 + lane;
+    const bool kvalid = (k < nlat_2);
+    const int kk = kvalid ? k : (nlat_2 - 1);
+
+    using C = Contract<PREC>;
+    double4_t acc[4][NCG];   // FP64 sums
+    typename C::acc_t part[4][NCG];   // FP32 partial sums over 16 steps (PREC > 0; unused at PREC 0)
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) {
+            acc[r][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
+            part[r][c] = C::zero();
+        }
+    auto flush = [&]() {
+        if constexpr (PREC > 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) {
+                    acc[r][c] += __builtin_convertvector(part[r][c], double4_t);
+                    part[r][c] = C::zero();
+                }
+        }
+    };
+
+    const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, vector_mode);
+    const int nsteps = (im <= imlim) ? o.nsteps : 0;
+    if (nsteps > 0) {
+        const int sofs = step_ofs[o.slot];
+        const int slot_rows = step_ofs[o.slot + 1] - sofs;   // > nsteps: the last rows of qmix are zero
+        const UniformTable2 abm(ab + sofs);   // uniform indices -> scalar loads
+        const double *qb = qmix + ((size_t)g * nsteps_total + sofs) * NCOL;
+        const double x2 = x2k[kk];
+        int i0 = kvalid ? i0tab[(size_t)o.slot * nlat_2 + kk] : nsteps;
+        double2 sd = z0tab[(size_t)o.slot * nlat_2 + kk];
+        if (o.sinv) {
+            const double inv = 1.0 / stk[kk];
+            sd.x *= inv;
+            sd.y *= inv;
+        }
+        if (i0 > nsteps) i0 = nsteps;
+        int wmin = i0, wmax = i0;
+        for (int off = 32; off >= 1; off >>= 1) {
+            wmin = min(wmin, __shfl_xor(wmin, off));
+            wmax = max(wmax, __shfl_xor(wmax, off));
+        }
+        // wave-uniform: keeps loop control and coefficient loads on the scalar unit
+        wmin = __builtin_amdgcn_readfirstlane(wmin);
+        wmax = __builtin_amdgcn_readfirstlane(wmax);
+        const int ibeg = wmin & ~3;
+        double za = 0.0, zb = 0.0;
+        if (i0 == ibeg) { za = sd.x; zb = sd.y; }
+
+        // advance 4 recurrence steps starting at step i, writing z_i..z_{i+3} to tile buffer
+        auto advance = [&](int i, int buf) {
+            double2 cf[4];
+#pragma unroll
+            for (int s = 0; s < 4; ++s) cf[s] = abm[min(i + s, nsteps - 1)];
+            if (i <= wmax) {   // some lane of this wave starts in this block
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    if (i + s == i0) { za = sd.x; zb = sd.y; }
+                    C::store_z(&ytile[wave][buf][s][lane], zb);
+                    const double zn = fma(fma(cf[s].x, x2, cf[s].y), zb, za);
+                    za = zb;
+                    zb = zn;
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; ++s) {
+                    C::store_z(&ytile[wave][buf][s][lane], zb);
+                    const double zn = fma(fma(cf[s].x, x2, cf[s].y), zb, za);
+                    za = zb;
+                    zb = zn;
+                }
+            }
+        };
+        auto load_b = [&](int i, double *b) {
+            const int ii = min(i + (lane >> 4), slot_rows - 1);   // rows >= nsteps hold zeros
+#pragma unroll
+            for (int c = 0; c < NCG; ++c) b[c] = qb[(size_t)ii * NCOL + c * 16 + (lane & 15)];
+        };
+
+        double bcur[NCG], bnxt[NCG];
+        advance(ibeg, 0);
+        load_b(ibeg, bcur);
+        int buf = 0, blk = 0;
+        for (int i = ibeg; i < nsteps; i += 4, ++blk) {
+            load_b(i + 4, bnxt);
+            __builtin_amdgcn_s_waitcnt(0xc07f);   // LDS tile of this block written
+            __builtin_amdgcn_wave_barrier();
+            typename C::a_t a[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a[r] = C::load_z(&ytile[wave][buf][lane >> 4][r * 16 + (lane & 15)]);
+#pragma unroll
+            for (int c = 0; c < NCG; ++c) {
+                const typename C::b_t bb = C::split(bcur[c]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if constexpr (PREC == 0) acc[r][c] = C::mma(a[r], bb, acc[r][c]);
+                    else part[r][c] = C::mma(a[r], bb, part[r][c]);
+                }
+            }
+            if (i + 4 < nsteps) advance(i + 4, buf ^ 1);
+            buf ^= 1;
+#pragma unroll
+            for (int c = 0; c < NCG; ++c) bcur[c] = bnxt[c];
+            if ((blk & 3) == 3) flush();
+        }
+        flush();
+    }
+
+    // D[row][col = l%16] (row from Contract::d_row); row tile r covers latitudes tile*64 + 16r + row.
+    const int col = lane & 15;
+    const int comp = col & 3;
+    const bool real_only = (im == 0);
+#pragma unroll
+    for (int c = 0; c < NCG; ++c) {
+        const int f = fbase + c * 4 + (col >> 2);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double mine = acc[r][c][j];
+                const double other = __shfl_xor(mine, 2);   // E <-> O partner
+                const int kr = tile * 64 + r * 16 + C::d_row(j, lane);
+                if (comp < 2 && kr < nlat_2 && f < nfields) {
+                    const double x = xk[kr];
+                    const double wmul = lat_weight ? wqk[kr] : 1.0;
+                    double n = (mine + x * other) * wmul;
+                    double sth = (mine - x * other) * wmul;
+                    if (real_only && comp == 1) { n = 0.0; sth = 0.0; }
+                    double *Ff = reinterpret_cast<double *>(F + (size_t)f * four_dist + (size_t)im * nlat);
+                    Ff[2 * kr + comp] = n;
+                    Ff[2 * (nlat - 1 - kr) + comp] = sth;
+                }
+            }
+        }
+    }
+}
+
+inline int synth_ncg(int nfields) { return nfields > 8 ? 4 : (nfields > 4 ? 2 : 1); }
+
+template <int PREC>
+void synth_tiles_launch(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
+                        int nsteps_total, int tile0, int ntl, hipStream_t s) {
+    const DeviceTables *t = a.t;
+    const int ntg = (ntl + kSynthWaves - 1) / kSynthWaves;
+    const int ncg = synth_ncg(a.nfields);
+    const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
+    dim3 grid((a.mmax + 1) * ntg, ng);
+#define SYNTH_LAUNCH(NC)                                                                                         \
+    hipLaunchKernelGGL((legendre_synth_kernel<NC, PREC>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq, \
+                       t->step_ofs, t->ab, t->i0, t->z0, qmix, nsteps_total, spec_ltop, F, four_dist, a.mmax,    \
+                       a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.lat_weight, tile0, ntl)
+    if (ncg == 4) SYNTH_LAUNCH(4);
+    else if (ncg == 2) SYNTH_LAUNCH(2);
+    else SYNTH_LAUNCH(1);
+#undef SYNTH_LAUNCH
+}
+
+// ---------------------------------------------------------------------------------------
+// Analysis: Q_lm = sum_k w_k P_lm(x_k) F_m(x_k)
+// Block = NW waves for one order im and 4*NCG fields; wave w owns latitude tiles
+// w + NW*r (r < NLR) of 64 pairs, lane = latitude pair.  The weighted E/O values of the
+// block's fields for the owned latitudes are held in registers as MFMA B operands.
+// Per 16-step block: each lane advances its recurrences into a wave-private LDS tile, the
+// tile is read back as the A operand (16 steps x 4 latitudes) and
+// D[16 step][16 col] += A * B runs over all owned K-groups; the waves' partial D go to a
+// double-buffered LDS area, one barrier, then every thread sums one (step, column) over
+// the waves and applies the post-mix (even: ue_i acc_i + ue1_{i-1} acc_{i-1}; odd:
+// uo_i acc_i) and stores Q.  Fixed summation order: results are deterministic.
+// ---------------------------------------------------------------------------------------
+template <int NW, int NLR, int NCG, int PREC>
+__global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
+    const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
+    const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
+    const double4 *__restrict__ pm, const int *__restrict__ i0tab, const double2 *__restrict__ z0tab,
+    const double2 *__restrict__ F, long four_dist, double2 *__restrict__ Q, long spec_dist, int lmax_spec,
+    int spec_ltop, int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
+    double mpos_scale, int lat_weight) {
+    constexpr int NCOL = 16 * NCG;
+    __shared__ double ytile[NW][16][65];
+    __shared__ double red[NW][16][NCOL + 1];   // single-buffered: two barriers per block, 2 blocks/CU fit
+    __shared__ double carry[2][NCOL];
+    __shared__ int s_ibeg[NW];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int im = blockIdx.x;
+    const int fbase = blockIdx.y * 4 * NCG;
+
+    const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, vector_mode);
+    const int nsteps = (im <= imlim) ? o.nsteps : 0;
+    const int sofs = step_ofs[o.slot];
+    const UniformTable2 abm(ab + sofs);   // uniform indices -> scalar loads
+    const double4 *pmm = pm + sofs;
+
+    double za[NLR], zb[NLR], x2v[NLR], sdx[NLR], sdy[NLR];
+    int i0v[NLR];
+    using C = Contract<PREC>;
+    typename C::b_t Ereg[NLR][NCG][16];
+    int my_ibeg = nsteps, my_imax = 0;
+#pragma unroll
+    for (int r = 0; r < NLR; ++r) {
+        const int k = (wave + NW * r) * 64 + lane;
+        const bool kv = (k < nlat_2);
+        const int kk = kv ? k : 0;
+        x2v[r] = x2k[kk];
+        int i0 = kv ? i0tab[(size_t)o.slot * nlat_2 + kk] : nsteps;
+        if (i0 > nsteps) i0 = nsteps;
+        double2 sd = z0tab[(size_t)o.slot * nlat_2 + kk];
+        if (o.sinv) {
+            const double inv = 1.0 / stk[kk];
+            sd.x *= inv;
+            sd.y *= inv;
+        }
+        i0v[r] = i0;
+        sdx[r] = sd.x;
+        sdy[r] = sd.y;
+        my_ibeg = min(my_ibeg, i0);
+        my_imax = max(my_imax, kv ? i0 : 0);
+        // B operand for K-group kg: latitude tile_base + 4*kg + lane/16, column lane%16 of group c
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) {
+            const int col = lane & 15;
+            const int f = fbase + 4 * c + (col >> 2);
+            const int comp = col & 3;
+#pragma unroll
+            for (int kg = 0; kg < 16; ++kg) {
+                const int kl = (wave + NW * r) * 64 + 4 * kg + (lane >> 4);
+                double v = 0.0;
+                if (kl < nlat_2 && f < nfields && nsteps > 0 && !(o.real_only && (comp & 1))) {
+                    const double2 *Ff = F + (size_t)f * four_dist + (size_t)im * nlat;
+                    const double2 fn = Ff[kl];
+                    const double2 fs = Ff[nlat - 1 - kl];
+                    const double w = lat_weight ? wqk[kl] : 1.0;
+                    if (comp == 0) v = (fn.x + fs.x) * w;
+                    else if (comp == 1) v = (fn.y + fs.y) * w;
+                    else if (comp == 2) v = (fn.x - fs.x) * w * xk[kl];
+                    else v = (fn.y - fs.y) * w * xk[kl];
+                }
+                Ereg[r][c][kg] = C::split(v);
+            }
+        }
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+        my_ibeg = min(my_ibeg, __shfl_xor(my_ibeg, off));
+        my_imax = max(my_imax, __shfl_xor(my_imax, off));
+    }
+    my_imax = __builtin_amdgcn_readfirstlane(my_imax);
+    if (lane == 0) s_ibeg[wave] = my_ibeg;
+    if (threadIdx.x < 2 * NCOL) (&carry[0][0])[threadIdx.x] = 0.0;
+    __syncthreads();
+    int ibeg = nsteps;
+    for (int w = 0; w < NW; ++w) ibeg = min(ibeg, s_ibeg[w]);
+    ibeg = __builtin_amdgcn_readfirstlane(ibeg) & ~15;   // block-uniform, scalar from here on
+#pragma unroll
+    for (int r = 0; r < NLR; ++r) {
+        za[r] = 0.0;
+        zb[r] = 0.0;
+        if (i0v[r] == ibeg) { za[r] = sdx[r]; zb[r] = sdy[r]; }
+    }
+
+    const double qscale = (im == 0) ? 1.0 : mpos_scale;
+    const int nsteps_out = (spec_ltop >= o.mfun) ? (spec_ltop - o.mfun) / 2 + 1 : 0;
+    double2 *Qbase = Q + o.spec_base;
+
+    // post-mix work split: thread handles elements e = threadIdx.x + j*64*NW (j < NE) of the
+    // 16 x NCOL block; their coefficient rows are prefetched before the MFMA phase.
+    constexpr int NE = (16 * NCOL + 64 * NW - 1) / (64 * NW);
+    int b = 0;
+    for (int i = ibeg; i < nsteps; i += 16, ++b) {
+        double pcx[NE], pcz[NE], pcp[NE];
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            const int e = threadIdx.x + j * 64 * NW;
+            const int row = min(e / NCOL, 15);
+            const int ii = min(i + row, nsteps - 1);
+            const double4 cf4 = pmm[ii];
+            pcx[j] = cf4.x;
+            pcz[j] = cf4.z;
+            pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
+        }
+        double2 cf[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
+        typename C::acc_t acc[NCG], acc2[NCG];   // two chains per column group (even / odd K-groups)
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) acc[c] = acc2[c] = C::zero();
+#pragma unroll
+        for (int r = 0; r < NLR; ++r) {
+            if (i <= my_imax) {   // some lane of this wave starts its recurrence here
+#pragma unroll
+                for (int s = 0; s < 16; ++s) {
+                    const int ii = i + s;
+                    if (ii == i0v[r]) { za[r] = sdx[r]; zb[r] = sdy[r]; }
+                    C::store_z(&ytile[wave][s][lane], zb[r]);
+                    const double zn = fma(fma(cf[s].x, x2v[r], cf[s].y), zb[r], za[r]);
+                    za[r] = zb[r];
+                    zb[r] = zn;
+                }
+            } else {
+#pragma unroll
+                for (int s = 0; s < 16; ++s) {
+                    C::store_z(&ytile[wave][s][lane], zb[r]);
+                    const double zn = fma(fma(cf[s].x, x2v[r], cf[s].y), zb[r], za[r]);
+                    za[r] = zb[r];
+                    zb[r] = zn;
+                }
+            }
+            __builtin_amdgcn_s_waitcnt(0xc07f);
+            __builtin_amdgcn_wave_barrier();
+            typename C::a_t a[16];
+#pragma unroll
+            for (int kg = 0; kg < 16; ++kg) a[kg] = C::load_z(&ytile[wave][lane & 15][4 * kg + (lane >> 4)]);
+            __builtin_amdgcn_s_waitcnt(0xc07f);   // all A operands in registers before the tile is rewritten
+            __builtin_amdgcn_wave_barrier();
+#pragma unroll
+            for (int kg = 0; kg < 16; kg += 2)
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) {
+                    acc[c] = C::mma(a[kg], Ereg[r][c][kg], acc[c]);
+                    acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
+                }
+        }
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) acc[c] += acc2[c];
+        // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
+        const int pb = b & 1;
+        __syncthreads();   // previous block's post-mix has finished reading red
+#pragma unroll
+        for (int c = 0; c < NCG; ++c)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) red[wave][C::d_row(j, lane)][16 * c + (lane & 15)] = acc[c][j];
+        __syncthreads();
+        // sum over waves + post-mix
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+            const int e = threadIdx.x + j * 64 * NW;
+            if (e >= 16 * NCOL) break;
+            const int row = e / NCOL, col = e - row * NCOL;
+            const int comp = col & 3;
+            const int f = fbase + (col >> 2);
+            double sum = 0.0, sprev = 0.0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) sum += red[w][row][col];
+            if (row == 15 && comp < 2) carry[pb][col] = sum;
+            const int ii = i + row;
+            if (f >= nfields || ii >= nsteps_out) continue;
+            const int le = o.mfun + 2 * ii;
+            double v = 0.0;
+            if (ii < nsteps) {
+                if (comp < 2) {
+                    if (row > 0) {
+#pragma unroll
+                        for (int w = 0; w < NW; ++w) sprev += red[w][row - 1][col];
+                    } else {
+                        sprev = carry[pb ^ 1][col];
+                    }
+                    v = pcx[j] * sum + pcp[j] * sprev;
+                } else {
+                    v = pcz[j] * sum;
+                }
+            }
+            const int ld = (comp < 2) ? le : le + 1;
+            if (ld <= spec_ltop) {
+                double out = (ld <= ltr) ? v * qscale : 0.0;
+                if (o.real_only && (comp & 1)) out = 0.0;
+                reinterpret_cast<double *>(&Qbase[(size_t)f * spec_dist + ld])[comp & 1] = out;
+            }
+        }
+    }
+    // degrees not reached by the step loop: before ibeg (negligible functions) and after nsteps
+    for (int ii = threadIdx.x; ii < nsteps_out; ii += 64 * NW) {
+        if (ii >= ibeg && ii < ibeg + ((nsteps - ibeg + 15) & ~15)) continue;
+        const int le = o.mfun + 2 * ii;
+        for (int f = fbase; f < min(nfields, fbase + 4 * NCG); ++f) {
+            double2 *Qf = Qbase + (size_t)f * spec_dist;
+            if (le <= spec_ltop) Qf[le] = make_double2(0.0, 0.0);
+            if (le + 1 <= spec_ltop) Qf[le + 1] = make_double2(0.0, 0.0);
+        }
+    }
+    if (vector_mode && im == 0 && threadIdx.x == 0) {   // degree 0 of the vector m=0 spectra
+        for (int f = fbase; f < min(nfields, fbase + 4 * NCG); ++f) Qbase[(size_t)f * spec_dist] = make_double2(0.0, 0.0);
+    }
+}
+
+template <int NW, int NLR, int NCG, int PREC>
+void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
+                 int spec_ltop, hipStream_t s) {
+    const DeviceTables *t = a.t;
+    dim3 grid(a.mmax + 1, (a.nfields + 4 * NCG - 1) / (4 * NCG));
+    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC>), grid, dim3(64 * NW), 0, s, t->x, t->x2, t->st, t->wq,
+                       t->step_ofs, t->ab, t->pm, t->i0, t->z0, F, four_dist, Q, spec_dist,
+                       spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
+                       a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale, a.lat_weight);
+}
+
+template <int PREC>
+void anal_launch(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
+                 int spec_ltop, hipStream_t s) {
+    const int ntiles = (a.nlat_2 + 63) / 64;
+    // waves per block: one latitude tile per wave up to 12 waves (measured fastest: lowest
+    // register count, 3 waves per SIMD); beyond, a divisor of the tile count, at most 8
+    int nw = 1;
+    if (ntiles <= 12) nw = ntiles;
+    else
+        for (int d = 8; d >= 1; --d)
+            if (ntiles % d == 0) { nw = d; break; }
+    static const int nw_env = getenv("SHTNS_AMD_ANAL_NW") ? atoi(getenv("SHTNS_AMD_ANAL_NW")) : 0;
+    if (nw_env > 0 && nw_env <= 16 && ntiles % nw_env == 0) nw = nw_env;
+    int nlr = ntiles / nw;
+    if (nlr > 6) { nw = 8; nlr = (ntiles + 7) / 8; }
+    static const int ncg_env = getenv("SHTNS_AMD_ANAL_NCG") ? atoi(getenv("SHTNS_AMD_ANAL_NCG")) : 0;
+    bool wide = (a.nfields > 4) && (nlr <= 3);   // 8 fields per block when registers allow
+    if (ncg_env == 1) wide = false;
+    if (ncg_env == 2 && nlr <= 3) wide = (a.nfields > 4);
+#define ANAL_CASE(W, R)                                                                              \
+    if (nw == W && nlr <= R) {                                                                       \
+        if (wide) launch_anal_t<W, R, 2, PREC>(a, F, four_dist, Q, spec_dist, spec_ltop, s);        \
+        else launch_anal_t<W, R, 1, PREC>(a, F, four_dist, Q, spec_dist, spec_ltop, s);             \
+        return;                                                                                      \
+    }
+    ANAL_CASE(1, 1) ANAL_CASE(1, 2) ANAL_CASE(1, 3) ANAL_CASE(1, 6)
+    ANAL_CASE(2, 1) ANAL_CASE(2, 3) ANAL_CASE(2, 6)
+    ANAL_CASE(3, 1) ANAL_CASE(3, 2) ANAL_CASE(3, 4) ANAL_CASE(3, 6)
+    ANAL_CASE(4, 1) ANAL_CASE(4, 2) ANAL_CASE(4, 3) ANAL_CASE(4, 6)
+    ANAL_CASE(5, 1) ANAL_CASE(5, 3) ANAL_CASE(5, 6)
+    ANAL_CASE(6, 1) ANAL_CASE(6, 2) ANAL_CASE(6, 3) ANAL_CASE(6, 6)
+    ANAL_CASE(7, 1) ANAL_CASE(7, 3) ANAL_CASE(7, 6)
+    ANAL_CASE(8, 1) ANAL_CASE(8, 2) ANAL_CASE(8, 3) ANAL_CASE(8, 4) ANAL_CASE(8, 6)
+    ANAL_CASE(9, 1) ANAL_CASE(10, 1) ANAL_CASE(11, 1) ANAL_CASE(12, 1)
+#undef ANAL_CASE
+    fprintf(stderr, "shtns_amd: no analysis kernel for nlat_2=%d\n", a.nlat_2);
+}
+
+}  // namespace shtns_amd
