@@ -103,11 +103,20 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed loop, copy a 512 MiB buffer once (reference kernel of known "
                          "traffic for hardware-counter runs)")
+    ap.add_argument("--workload", choices=["pair", "uneven", "vector"], default="pair",
+                    help="pair: the headline scalar pair at lmax=1023 (default); uneven: 256 transforms "
+                         "of uneven lmax from one graph plus one lmax=4095 field; vector: spheroidal/"
+                         "toroidal pairs at lmax=511")
+    ap.add_argument("--tune", action="store_true", help="uneven workload: autotune each configuration first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is described in DESIGN.md but not built.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "uneven":
+        return bench_uneven(args)
+    if args.workload == "vector":
+        return bench_vector(args)
 
     import numpy as np
     import torch
@@ -235,6 +244,155 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
+
+
+UNEVEN_LMAX = (63, 127, 255, 511, 1023)
+UNEVEN_N, UNEVEN_SEED_L, UNEVEN_SEED_DATA = 256, 4, 5
+
+
+def uneven_lmaxes():
+    """lmax of the 256 transforms: uniform over UNEVEN_LMAX from SplitMix64 (seed 4)."""
+    from shtns_amd.synthetic import splitmix64
+    u = splitmix64(UNEVEN_SEED_L, UNEVEN_N)
+    return [UNEVEN_LMAX[int(x % len(UNEVEN_LMAX))] for x in u]
+
+
+def bench_uneven(args):
+    """256 scalar transform pairs of uneven lmax, Gauss grids nlat = 3(lmax+1)/2, nphi = 2 nlat,
+    one HIP graph per direction (cushtns_batch_*), FP64 and FP32-split; then one lmax=4095
+    field on 6144 x 12288 (FP64)."""
+    import torch
+    import shtns_amd
+    from shtns_amd import _lib
+    from shtns_amd.batch import UnevenBatch, tune
+    from shtns_amd.synthetic import spectral_field
+    lmaxes = uneven_lmaxes()
+    cfg = {}
+    for L in UNEVEN_LMAX:
+        nlat = 3 * (L + 1) // 2
+        sh = shtns_amd.sht(L)
+        sh.set_grid(nlat, 2 * nlat, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+        cfg[L] = sh
+    shts = [cfg[L] for L in lmaxes]
+    alg = sum(2 * (leg_flops(L, 3 * (L + 1) // 2) + fft_flops(3 * (L + 1) // 2, 3 * (L + 1))) for L in lmaxes)
+    out = {}
+    results = {}
+    for name, prec in (("fp64", shtns_amd.SHT_PREC_F64), ("fp32_split", shtns_amd.SHT_PREC_SPLIT)):
+        b = UnevenBatch(shts, prec=prec, tune=args.tune)
+        qs = []
+        for i, L in enumerate(lmaxes):
+            q = torch.from_numpy(spectral_field(L, L, seed=UNEVEN_SEED_DATA + i))
+            b.arrays(i)[0].copy_(q)
+            qs.append(q)
+        q0 = [b.arrays(i)[0].clone() for i in range(len(b))]
+        for _ in range(args.warmup):
+            b.synth()
+            b.analys()
+        torch.cuda.synchronize()
+        # restore inputs (the round trips of the warm-up rewrote them with their own output)
+        for i in range(len(b)):
+            b.arrays(i)[0].copy_(q0[i])
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b.synth()
+            b.analys()
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        _lib.check_error("uneven batch")
+        results[name] = [b.arrays(i)[0].clone() for i in range(len(b))]
+        num = sum(float(torch.linalg.norm(results[name][i] - q0[i]) ** 2) for i in range(len(b)))
+        den = sum(float(torch.linalg.norm(q0[i]) ** 2) for i in range(len(b)))
+        out[name] = {"pairs_per_s": round(UNEVEN_N * args.steps / el, 2), "ms_per_step": round(el / args.steps * 1e3, 4),
+                     "round_trip_rel_l2": (num / den) ** 0.5,
+                     "fp64_frac_of_peak": round(alg * args.steps / el / 1e12 / FP64_PEAK_TFLOPS, 4)}
+        b.close()
+    num = sum(float(torch.linalg.norm(results["fp32_split"][i] - results["fp64"][i]) ** 2) for i in range(UNEVEN_N))
+    den = sum(float(torch.linalg.norm(results["fp64"][i]) ** 2) for i in range(UNEVEN_N))
+    out["fp32_split"]["rel_l2_vs_fp64"] = (num / den) ** 0.5
+    out["fp32_split"]["speedup_vs_fp64"] = round(out["fp32_split"]["pairs_per_s"] / out["fp64"]["pairs_per_s"], 3)
+    # one lmax = 4095 field
+    L4, n4 = 4095, 6144
+    s4 = shtns_amd.sht(L4)
+    s4.set_grid(n4, 2 * n4, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+    q4 = torch.from_numpy(spectral_field(L4, L4, seed=UNEVEN_SEED_DATA + UNEVEN_N)).cuda()
+    v4 = s4.synth(q4)
+    s4.analys(v4)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    v4 = s4.synth(q4)
+    r4 = s4.analys(v4)
+    torch.cuda.synchronize()
+    el4 = time.perf_counter() - t0
+    f4 = 2 * (leg_flops(L4, n4) + fft_flops(n4, 2 * n4))
+    out["lmax4095"] = {"pair_ms": round(el4 * 1e3, 3), "round_trip_rel_l2": float(torch.linalg.norm(r4 - q4) / torch.linalg.norm(q4)),
+                       "fp64_frac_of_peak": round(f4 / el4 / 1e12 / FP64_PEAK_TFLOPS, 4)}
+    counts = {L: lmaxes.count(L) for L in UNEVEN_LMAX}
+    print(json.dumps({
+        "metric": "uneven batch: SH_to_spat+spat_to_SH pairs/s, 256 transforms with lmax in {63..1023}",
+        "value": out["fp64"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out["fp64"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64/Box-Muller a_lm ~ N(0,(l+1)^-2), lmax draw seed 4, data seed 5+i)",
+        "config": {"workload": "256 scalar transforms, lmax uniform in {63,127,255,511,1023}, Gauss nlat=3(lmax+1)/2, "
+                               "nphi=2 nlat, theta-contiguous, one HIP graph per direction; plus one lmax=4095 "
+                               "6144x12288 FP64 field", "counts": counts, "tuned": bool(args.tune),
+                   "parallelism": "single GPU"},
+        "legs": out}))
+
+
+def bench_vector(args):
+    """32 spheroidal/toroidal pairs at lmax=511 on 768 x 1536 (SHsphtor_to_spat then
+    spat_to_SHsphtor), S,T ~ N(0,(l+1)^-3) per component, seed 2; FP64 and FP32 contraction."""
+    import torch
+    import shtns_amd
+    from shtns_amd import _lib
+    from shtns_amd.synthetic import spectral_batch
+    L, nlat, nphi, nb = 511, 768, 1536, 32
+    sh = shtns_amd.sht(L)
+    sh.set_grid(nlat, nphi, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+    S = torch.from_numpy(spectral_batch(L, L, nb, seed=2, power=3.0)).cuda()
+    T = torch.from_numpy(spectral_batch(L, L, nb, seed=2 + nb, power=3.0)).cuda()
+    S[:, 0] = 0
+    T[:, 0] = 0
+    nlm1 = (L + 2) * (L + 3) // 2
+    alg = 2 * (2 * 2 * nlat * nlm1 + 2 * fft_flops(nlat, nphi)) * nb
+    out = {}
+    ref = None
+    for name, prec in (("fp64", shtns_amd.SHT_PREC_F64), ("fp32_split", shtns_amd.SHT_PREC_SPLIT),
+                       ("fp32", shtns_amd.SHT_PREC_F32)):
+        sh.set_precision(prec)
+        for _ in range(args.warmup):
+            vt, vp = sh.synth(S, T)
+            s2, t2 = sh.analys(vt, vp)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            vt, vp = sh.synth(S, T)
+            s2, t2 = sh.analys(vt, vp)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        _lib.check_error("vector")
+        err = float(torch.sqrt((torch.linalg.norm(s2 - S) ** 2 + torch.linalg.norm(t2 - T) ** 2) /
+                               (torch.linalg.norm(S) ** 2 + torch.linalg.norm(T) ** 2)))
+        leg = {"pairs_per_s": round(nb * args.steps / el, 2), "ms_per_step": round(el / args.steps * 1e3, 4),
+               "round_trip_rel_l2": err, "fp64_frac_of_peak": round(alg * args.steps / el / 1e12 / FP64_PEAK_TFLOPS, 4)}
+        if ref is None:
+            ref = (vt.clone(), vp.clone())
+        else:
+            leg["synth_rel_l2_vs_fp64"] = float(torch.sqrt((torch.linalg.norm(vt - ref[0]) ** 2 + torch.linalg.norm(vp - ref[1]) ** 2) /
+                                                           (torch.linalg.norm(ref[0]) ** 2 + torch.linalg.norm(ref[1]) ** 2)))
+        out[name] = leg
+    sh.set_precision(shtns_amd.SHT_PREC_AUTO)
+    print(json.dumps({
+        "metric": "vector SHsphtor_to_spat+spat_to_SHsphtor pairs/s at lmax=511",
+        "value": out["fp64"]["pairs_per_s"], "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out["fp64"]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (SplitMix64/Box-Muller S,T ~ N(0,(l+1)^-3), seeds 2+f, 34+f; S_00=T_00=0)",
+        "config": {"workload": "32 (S,T) pairs, lmax=mmax=511, Gauss 768x1536 theta-contiguous", "lmax": L,
+                   "nlat": nlat, "nphi": nphi, "batch": nb, "parallelism": "single GPU"},
+        "legs": out}))
 
 
 if __name__ == "__main__":

@@ -66,6 +66,35 @@ void cu_spat_to_SH_float_batch(shtns_cfg, float *Vr, cplx_f *Qlm, int ltr, int n
 #define SHT_PREC_SPLIT 2
 int cushtns_set_precision(shtns_cfg, int prec);
 
+/* Autotuner (additive, no SHTns counterpart).  Times the candidate kernel shapes of this
+ * configuration on a batch of nfields fields -- synthesis field groups per wave, analysis
+ * waves per block and field groups per block, register or LDS-staged FFT -- and keeps the
+ * fastest for later calls on this configuration (clones made afterwards inherit them).
+ * Allocates and frees temporary device buffers; synchronizes the compute stream.
+ * cushtns_tuning reads the five choices (0 = automatic):
+ * {synthesis groups, analysis waves, analysis groups, synthesis FFT, analysis FFT}, with FFT
+ * 1 = register kernel, 2 = LDS-staged kernel.  Both return 0, or -1 on error. */
+int cushtns_tune(shtns_cfg, int nfields);
+int cushtns_tuning(shtns_cfg, int *choices);
+
+/* Uneven batches (additive, no SHTns counterpart): n scalar transforms, transform i on
+ * configuration cfgs[i] (any mix of lmax / grids, grids already set).  The batch owns the
+ * device arrays: transforms sharing a configuration are stored contiguously and run as one
+ * batched launch, and the groups of different configurations run as parallel branches of
+ * one HIP graph, captured when the batch is created and replayed by each call.
+ *   prec  contraction precision of every transform (SHT_PREC_*);
+ *   tune  nonzero: run cushtns_tune for each group's configuration and batch size first.
+ * The batch clones the configurations (their streams and tuning are not changed).
+ * cushtns_batch_arrays gives the device arrays of transform i: Qlm[nlm] (spectral) and
+ * Vr[nspat] (spatial, that configuration's layout).  The transforms read and write them;
+ * the calls enqueue the graph on `stream` and return without synchronizing. */
+typedef struct shtns_batch_info *shtns_batch;
+shtns_batch cushtns_batch_create(int n, const shtns_cfg *cfgs, int prec, int tune);
+int cushtns_batch_arrays(shtns_batch, int i, cplx **Qlm, double **Vr);
+int cushtns_batch_SH_to_spat(shtns_batch, hipStream_t stream);
+int cushtns_batch_spat_to_SH(shtns_batch, hipStream_t stream);
+void cushtns_batch_destroy(shtns_batch);
+
 /* Adjoint (transpose) operators, for reverse-mode differentiation (additive).
  *   cu_SH_to_spat_adjoint:  Qlm = S^T(Vr)   with S the synthesis operator of SH_to_spat
  *   cu_spat_to_SH_adjoint:  Vr  = A^T(Qlm)  with A the analysis operator of spat_to_SH

@@ -334,8 +334,7 @@ __global__ __launch_bounds__(WaveGeom<M>::NT) void fft_anal_wave(const TV *__res
 
 template <class K>
 void lds_limit(K kern, size_t lds) {
-    if (lds > 65536)
-        (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    ensure_lds_limit((const void *)kern, lds);
 }
 
 }  // namespace

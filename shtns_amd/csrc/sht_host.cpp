@@ -33,7 +33,8 @@ static int g_gpu_request = 0;              // device chosen by shtns_use_gpu (-1
 static thread_local int g_last_err = 0;
 static thread_local char g_last_msg[256] = "";
 
-static void set_error(int code, const char *fmt, ...) {
+namespace shtns_amd {
+void set_error(int code, const char *fmt, ...) {
     g_last_err = code;
     va_list ap;
     va_start(ap, fmt);
@@ -41,10 +42,11 @@ static void set_error(int code, const char *fmt, ...) {
     va_end(ap);
     if (code) fprintf(stderr, "*** [shtns_amd] %s\n", g_last_msg);
 }
-static void clear_error() {
+void clear_error() {
     g_last_err = 0;
     g_last_msg[0] = 0;
 }
+}  // namespace shtns_amd
 
 // invalid sizes abort the program, as SHTns's shtns_runerr does (sht_init.c:68-72)
 [[noreturn]] static void runerr(const char *msg) {
@@ -681,6 +683,9 @@ static LegendreArgs leg_args(shtns_cfg c, int ltr, int imlim, int nfields) {
     a.vector_mode = 0;
     a.mpos_scale = 1.0;
     a.lat_weight = 0;
+    a.synth_ncg = c->tune_synth_ncg;
+    a.anal_nw = c->tune_anal_nw;
+    a.anal_ncg = c->tune_anal_ncg;
     return a;
 }
 
@@ -798,7 +803,7 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
     const size_t fbytes = (size_t)nfields * four_dist * sizeof(double2);
-    const size_t qbytes = synth_qmix_doubles(nfields, c->nsteps_total) * sizeof(double);
+    const size_t qbytes = synth_qmix_doubles(nfields, c->nsteps_total, c->tune_synth_ncg) * sizeof(double);
     char *base = (char *)scratch(c, fbytes + qbytes);
     if (!base) return;
     double2 *F = (double2 *)base;
@@ -825,7 +830,8 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
         }
         ProfScope ps(c, PK_FFT_SYNTH);
         if (!vf) {
-            launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S);
+            launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S, 0,
+                             -1, c->tune_fft_synth);
         } else if (!vstage) {
             launch_fft_synth_wave(&c->d, c->nlat, c->nphi / 2, imlim, nfields, F, four_dist, vf, spat_dist, sp, st, S,
                                   0, c->nlat);
@@ -883,7 +889,7 @@ static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfield
             ProfScope ps(c, PK_FFT_ANAL);
             if (!vf) {
                 launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st,
-                                S);
+                                S, 0, -1, c->tune_fft_anal);
             } else if (!vstage) {
                 launch_fft_anal_wave(&c->d, c->nlat, c->nphi / 2, c->mmax, imlim, nfields, vf, spat_dist, F,
                                      four_dist, sp, st, S, 0, c->nlat);
@@ -935,7 +941,7 @@ static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, d
     const long vw_dist = nlm_calc(c->lmax + 1, c->mmax, c->mres) + 2;
     const size_t fbytes = (size_t)2 * nfields * four_dist * sizeof(double2);
     const size_t vbytes = (size_t)2 * nfields * vw_dist * sizeof(double2);
-    const size_t qbytes = synth_qmix_doubles(2 * nfields, c->nsteps_total) * sizeof(double);
+    const size_t qbytes = synth_qmix_doubles(2 * nfields, c->nsteps_total, c->tune_synth_ncg) * sizeof(double);
     char *base = (char *)scratch(c, fbytes + vbytes + qbytes);
     if (!base) return;
     double2 *F = (double2 *)base;

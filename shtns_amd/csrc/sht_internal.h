@@ -92,6 +92,8 @@ struct shtns_info {
     // staging for the FP32-data entry points (FP64 copies of spectra / spatial fields)
     void *xscratch = nullptr;
     size_t xscratch_bytes = 0;
+    // kernel shapes picked by cushtns_tune (0: automatic)
+    int tune_synth_ncg = 0, tune_anal_nw = 0, tune_anal_ncg = 0, tune_fft_synth = 0, tune_fft_anal = 0;
 };
 
 namespace shtns_amd {
@@ -105,7 +107,15 @@ struct LegendreArgs {
     double mpos_scale;        // factor on m>0 orders (adjoints use 2 or 0.5)
     int lat_weight;           // synthesis: multiply output by wq_k; analysis: 0 => unit weights
     int prec = 0;             // contraction precision: 0 FP64, 1 FP32, 2 FP32 hi/lo split (sht_legendre.h)
+    // kernel shapes (0: automatic choice; set by the autotuner, cushtns_tune)
+    int synth_ncg = 0;        // synthesis field groups of 4 per wave: 1, 2 or 4
+    int anal_nw = 0;          // analysis waves per block (divides the number of latitude tiles)
+    int anal_ncg = 0;         // analysis field groups of 4 per block: 1 or 2
 };
+
+// status of the last C-API call of this thread (shtns_last_error); set_error also prints
+void set_error(int code, const char *fmt, ...);
+void clear_error();
 
 // Contraction precisions of the Legendre kernels (SHT_PREC_* in shtns_hip.h)
 constexpr int kPrecF64 = 0, kPrecF32 = 1, kPrecSplit = 2;
@@ -114,18 +124,21 @@ constexpr int kPrecF64 = 0, kPrecF32 = 1, kPrecSplit = 2;
 // qmix: scratch of synth_qmix_doubles(nfields, nsteps_total) doubles (pre-mixed coefficients)
 void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
                            double2 *F, long four_dist, double *qmix, int nsteps_total, hipStream_t s);
-size_t synth_qmix_doubles(int nfields, int nsteps_total);
+size_t synth_qmix_doubles(int nfields, int nsteps_total, int want_ncg = 0);
+// analysis block shape (waves, latitude tiles per wave) with a compiled kernel
+bool anal_shape_ok(int nw, int nlr);
 void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dist,
                           double2 *Qlm, long spec_dist, int spec_ltop, hipStream_t s);
 // Spatial element (ip, it) of a field sits at ip*sp + it*st (theta-contiguous: sp=nlat, st=1).
 // lat_base / nlat_chunk restrict the transform to latitudes [lat_base, lat_base + nlat_chunk)
 // (nlat_chunk < 0: all latitudes).
+// kind: 0 automatic, 1 register FFT, 2 LDS-staged FFT (where a kernel for the length exists)
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                       const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s,
-                      int lat_base = 0, int nlat_chunk = -1);
+                      int lat_base = 0, int nlat_chunk = -1, int kind = 0);
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                      const double *V, long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s,
-                     int lat_base = 0, int nlat_chunk = -1);
+                     int lat_base = 0, int nlat_chunk = -1, int kind = 0);
 // Register-resident FFT kernels (sht_fft.hip) for M = nphi/2 in {16, 24} x 2^k up to 6144,
 // latitudes [lat_base, lat_end); return false (nothing launched) for other M.
 bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
@@ -142,6 +155,9 @@ bool launch_fft_anal_wave(const DeviceTables *t, int nlat, int M, int mmax, int 
                           long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
                           int lat_end);
 bool fft_wave_supported(int M);
+bool lds_fft_specialized(int M);   // an LDS-staged FFT kernel compiled for this length
+// raise a kernel's dynamic LDS limit once (outside any launch sequence that may be captured)
+void ensure_lds_limit(const void *kern, size_t lds);
 // element-wise FP32 <-> FP64 copies of nfields arrays of n scalars (sht_kernels.hip)
 void launch_convert(const float *src, long src_dist, double *dst, long dst_dist, long n, int nfields, hipStream_t s);
 void launch_convert(const double *src, long src_dist, float *dst, long dst_dist, long n, int nfields, hipStream_t s);

@@ -22,6 +22,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <unordered_map>
+#include <mutex>
 
 #include "sht_legendre.h"
 
@@ -161,8 +163,14 @@ __global__ __launch_bounds__(256) void premix_kernel(const int *__restrict__ ste
     }
 }
 
-size_t synth_qmix_doubles(int nfields, int nsteps_total) {
-    const int ncg = synth_ncg(nfields);
+bool anal_shape_ok(int nw, int nlr) {
+    // the (waves, tiles per wave) cases compiled in anal_launch (sht_legendre.h)
+    static const int maxr[13] = {0, 6, 6, 6, 6, 6, 6, 6, 6, 1, 1, 1, 1};
+    return nw >= 1 && nw <= 12 && nlr >= 1 && nlr <= maxr[nw];
+}
+
+size_t synth_qmix_doubles(int nfields, int nsteps_total, int want_ncg) {
+    const int ncg = synth_ncg(nfields, want_ncg);
     const int ng = (nfields + 4 * ncg - 1) / (4 * ncg);
     return (size_t)ng * nsteps_total * 16 * ncg;
 }
@@ -170,7 +178,7 @@ size_t synth_qmix_doubles(int nfields, int nsteps_total) {
 void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop, double *qmix,
                    int nsteps_total, hipStream_t s) {
     const DeviceTables *t = a.t;
-    const int ncg = synth_ncg(a.nfields);
+    const int ncg = synth_ncg(a.nfields, a.synth_ncg);
     const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
     dim3 grid(a.mmax + 1, ng, 4);
     hipLaunchKernelGGL(premix_kernel, grid, dim3(256), 0, s, t->step_ofs, t->pm, Qlm, spec_dist, spec_ltop, qmix,
@@ -475,7 +483,7 @@ static constexpr int fft_tc(int M) { return (32L * M <= 65536) ? 2 : 1; }
 #define FFT_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536) X(2048) \
     X(3072) X(4096) X(6144)
 
-static bool lds_fft_specialized(int M) {
+bool lds_fft_specialized(int M) {
 #define LDS_OK(MM) if (M == MM) return true;
     FFT_SIZES(LDS_OK)
 #undef LDS_OK
@@ -486,26 +494,41 @@ static bool lds_fft_specialized(int M) {
 // wave, so its spatial accesses are contiguous only in the phi-contiguous layout (sp == 1);
 // in the theta-contiguous layout the LDS kernel, whose blocks interleave two latitudes, is
 // faster where a specialized one exists.  SHTNS_AMD_FFT=wave|lds forces one of them.
-static bool use_wave_fft(int M, long sp) {
+static bool use_wave_fft(int M, long sp, int kind) {
     static const char *env = getenv("SHTNS_AMD_FFT");
+    if (kind == 1) return true;
+    if (kind == 2 && lds_fft_specialized(M)) return false;
     if (env && !strcmp(env, "lds")) return false;
     if (env && !strcmp(env, "wave")) return true;
     return sp == 1 || !lds_fft_specialized(M);
 }
 
+// hipFuncSetAttribute once per kernel and size: kept out of the launch path so launches can
+// be captured into HIP graphs
+void ensure_lds_limit(const void *kern, size_t lds) {
+    if (lds <= 65536) return;
+    static std::mutex mu;
+    static std::unordered_map<const void *, size_t> done;
+    std::lock_guard<std::mutex> g(mu);
+    size_t &have = done[kern];
+    if (have >= lds) return;
+    (void)hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    have = lds;
+}
+
 template <class K>
 static void set_lds_limit(K kern, size_t lds) {
-    if (lds > 65536) (void)hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    ensure_lds_limit((const void *)kern, lds);
 }
 
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                       const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s,
-                      int lat_base, int nlat_chunk) {
+                      int lat_base, int nlat_chunk, int kind) {
     const int M = nphi / 2;
     if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
-    if (!generic && use_wave_fft(M, sp) &&
+    if (!generic && use_wave_fft(M, sp, kind) &&
         launch_fft_synth_wave(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end))
         return;
 #define FFT_SYN_CASE(MM)                                                                                        \
@@ -530,12 +553,12 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
 
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields, const double *V,
                      long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
-                     int nlat_chunk) {
+                     int nlat_chunk, int kind) {
     const int M = nphi / 2;
     if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
-    if (!generic && use_wave_fft(M, sp) &&
+    if (!generic && use_wave_fft(M, sp, kind) &&
         launch_fft_anal_wave(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end))
         return;
 #define FFT_ANA_CASE(MM)                                                                                          \

@@ -285,14 +285,17 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     }
 }
 
-inline int synth_ncg(int nfields) { return nfields > 8 ? 4 : (nfields > 4 ? 2 : 1); }
+inline int synth_ncg(int nfields, int want = 0) {
+    if (want == 1 || want == 2 || want == 4) return want;
+    return nfields > 8 ? 4 : (nfields > 4 ? 2 : 1);
+}
 
 template <int PREC>
 void synth_tiles_launch(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
                         int nsteps_total, int tile0, int ntl, hipStream_t s) {
     const DeviceTables *t = a.t;
     const int ntg = (ntl + kSynthWaves - 1) / kSynthWaves;
-    const int ncg = synth_ncg(a.nfields);
+    const int ncg = synth_ncg(a.nfields, a.synth_ncg);
     const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
     dim3 grid((a.mmax + 1) * ntg, ng);
 #define SYNTH_LAUNCH(NC)                                                                                         \
@@ -556,12 +559,14 @@ void anal_launch(const LegendreArgs &a, const double2 *F, long four_dist, double
             if (ntiles % d == 0) { nw = d; break; }
     static const int nw_env = getenv("SHTNS_AMD_ANAL_NW") ? atoi(getenv("SHTNS_AMD_ANAL_NW")) : 0;
     if (nw_env > 0 && nw_env <= 16 && ntiles % nw_env == 0) nw = nw_env;
+    if (a.anal_nw > 0 && ntiles % a.anal_nw == 0 && anal_shape_ok(a.anal_nw, ntiles / a.anal_nw)) nw = a.anal_nw;
     int nlr = ntiles / nw;
     if (nlr > 6) { nw = 8; nlr = (ntiles + 7) / 8; }
     static const int ncg_env = getenv("SHTNS_AMD_ANAL_NCG") ? atoi(getenv("SHTNS_AMD_ANAL_NCG")) : 0;
+    const int ncg_want = a.anal_ncg ? a.anal_ncg : ncg_env;
     bool wide = (a.nfields > 4) && (nlr <= 3);   // 8 fields per block when registers allow
-    if (ncg_env == 1) wide = false;
-    if (ncg_env == 2 && nlr <= 3) wide = (a.nfields > 4);
+    if (ncg_want == 1) wide = false;
+    if (ncg_want == 2 && nlr <= 3) wide = (a.nfields > 4);
 #define ANAL_CASE(W, R)                                                                              \
     if (nw == W && nlr <= R) {                                                                       \
         if (wide) launch_anal_t<W, R, 2, PREC>(a, F, four_dist, Q, spec_dist, spec_ltop, s);        \
