@@ -73,7 +73,9 @@ int cushtns_set_precision(shtns_cfg, int prec);
  * Allocates and frees temporary device buffers; synchronizes the compute stream.
  * cushtns_tuning reads the five choices (0 = automatic):
  * {synthesis groups, analysis waves, analysis groups, synthesis FFT, analysis FFT}, with FFT
- * 1 = register kernel, 2 = LDS-staged kernel.  Both return 0, or -1 on error. */
+ * 1 = register kernel, 2 = LDS-staged kernel, 3 / 4 = block-cooperative register kernel with
+ * 256 / 128 threads per block.
+ * Both return 0, or -1 on error. */
 int cushtns_tune(shtns_cfg, int nfields);
 int cushtns_tuning(shtns_cfg, int *choices);
 

@@ -78,8 +78,14 @@ extern "C" int cushtns_tune(shtns_cfg c, int nfields) {
     cplx *Q = (cplx *)q.p;
     double *V = (double *)v.p;
     const int M = c->nphi / 2;
-    std::vector<int> fft_kinds{0};
-    if (fft_wave_supported(M) && lds_fft_specialized(M)) fft_kinds = {1, 2};
+    std::vector<int> fft_kinds;
+    if (fft_wave_supported(M)) fft_kinds.push_back(1);
+    if (lds_fft_specialized(M)) fft_kinds.push_back(2);
+    if (fft_coop_supported(M)) {
+        fft_kinds.push_back(3);
+        fft_kinds.push_back(4);
+    }
+    if (fft_kinds.size() < 2) fft_kinds = {0};
 
     // synthesis: field groups per wave x FFT kernel
     std::vector<int> ncgs{1};

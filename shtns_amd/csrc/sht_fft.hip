@@ -332,6 +332,149 @@ __global__ __launch_bounds__(WaveGeom<M>::NT) void fft_anal_wave(const TV *__res
     }
 }
 
+// ---- block-cooperative variant for the theta-contiguous layout ---------------------------
+// The register kernels above give each latitude its own lanes, so in the theta-contiguous
+// layout (consecutive latitudes adjacent in memory) every lane's load or store touches a
+// different cache line.  Here a block of 256 threads holds T = 256/LQ consecutive latitudes
+// and moves data between HBM and its LDS sequence buffers with the latitude index fastest
+// across threads (runs of T*16 B of F and T*8 B of V per cache line), then each sequence's
+// lanes run the same Stockham passes as above.  LDS: T sequence buffers (256 R 16 B).
+template <int M, int NTB>
+struct CoopGeom {
+    using G = WaveGeom<M>;
+    static constexpr int LQ = G::LQ, R = G::R;
+    static constexpr int NT = NTB;       // threads per block (128 or 256)
+    static constexpr int T = NT / LQ;   // latitudes per block
+    static constexpr size_t lds_bytes = (size_t)T * lds_seq_stride(M) * sizeof(double2);
+};
+
+template <int M, int NTB>
+__global__ __launch_bounds__(NTB) void fft_synth_coop(const double2 *__restrict__ F, long four_dist,
+                                                      double *__restrict__ V, long spat_dist,
+                                                      const double2 *__restrict__ roots, int nlat, int imlim, long sp,
+                                                      long stt, int lat_base, int lat_end) {
+    using CG = CoopGeom<M, NTB>;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = lds_seq_stride(M);
+    extern __shared__ double2 lds[];
+    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
+    // 1) X_k of the block's T latitudes into LDS, latitude fastest
+    {
+        // R loads per thread, all issued before the first LDS store
+        const int tl = threadIdx.x % T, row = threadIdx.x / T;
+        const bool live = lat0 + tl < lat_end;
+        const double2 *Ff = F + (size_t)blockIdx.y * four_dist + (live ? lat0 + tl : 0);
+        double2 *dst = lds + tl * STR;
+        double2 xr[R];
+        sfor<R>([&](auto ic) {
+            const int k = row + (NT / T) * decltype(ic)::value;
+            xr[decltype(ic)::value] = (live && k <= imlim) ? Ff[(size_t)k * nlat] : make_double2(0.0, 0.0);
+        });
+        sfor<R>([&](auto ic) { dst[lds_slot(row + (NT / T) * decltype(ic)::value)] = xr[decltype(ic)::value]; });
+    }
+    __syncthreads();
+    const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
+    double2 *seq = lds + sq * STR;
+    const double2 wl = roots[l];
+    double2 v[R];
+    sfor<R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int k = l + LQ * j, km = M - k;
+        const double2 xk = (k <= imlim) ? seq[lds_slot(k)] : make_double2(0.0, 0.0);
+        const double2 xm = (k > 0 && km <= imlim) ? seq[lds_slot(km)] : make_double2(0.0, 0.0);
+        const double2 A = c_add(xk, c_conj(xm)), B = c_sub(xk, c_conj(xm));
+        const double2 w = rot96<(96 / (2 * R)) * j, 1>(wl);
+        const double2 wB = c_mul(w, B);
+        v[j] = make_double2(A.x - wB.y, A.y + wB.x);
+    });
+    __syncthreads();
+    wave_passes<M, 1, 0>(v, seq, l, roots);
+    // 2) z_n to LDS, then V out with the latitude fastest
+    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value]; });
+    __syncthreads();
+    {
+        const int tl = threadIdx.x % T, row = threadIdx.x / T;
+        if (lat0 + tl < lat_end) {
+            double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(lat0 + tl) * stt;
+            const double2 *src = lds + tl * STR;
+            sfor<R>([&](auto ic) {
+                const int n = row + (NT / T) * decltype(ic)::value;
+                const double2 z = src[lds_slot(n)];
+                Vf[(size_t)(2 * n) * sp] = z.x;
+                Vf[(size_t)(2 * n + 1) * sp] = z.y;
+            });
+        }
+    }
+}
+
+template <int M, int NTB>
+__global__ __launch_bounds__(NTB) void fft_anal_coop(const double *__restrict__ V, long spat_dist,
+                                                     double2 *__restrict__ F, long four_dist,
+                                                     const double2 *__restrict__ roots, int nlat, int mmax, int imlim,
+                                                     long sp, long stt, int lat_base, int lat_end) {
+    using CG = CoopGeom<M, NTB>;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = lds_seq_stride(M);
+    extern __shared__ double2 lds[];
+    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
+    // 1) z_n of the block's T latitudes into LDS, latitude fastest
+    {
+        const int tl = threadIdx.x % T, row = threadIdx.x / T;
+        const bool live = lat0 + tl < lat_end;
+        const double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(live ? lat0 + tl : 0) * stt;
+        double2 *dst = lds + tl * STR;
+        double2 zr[R];
+        sfor<R>([&](auto ic) {
+            const long n = row + (NT / T) * decltype(ic)::value;
+            zr[decltype(ic)::value] = live ? make_double2(Vf[(2 * n) * sp], Vf[(2 * n + 1) * sp]) : make_double2(0.0, 0.0);
+        });
+        sfor<R>([&](auto ic) { dst[lds_slot(row + (NT / T) * decltype(ic)::value)] = zr[decltype(ic)::value]; });
+    }
+    __syncthreads();
+    const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
+    double2 *seq = lds + sq * STR;
+    double2 v[R];
+    sfor<R>([&](auto jc) { v[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+    __syncthreads();
+    const double2 wl = roots[l];
+    wave_passes<M, -1, 0>(v, seq, l, roots);
+    // 2) Z_n to LDS (Z_{M-k} sits in another lane), X_k in registers, X_k back to LDS
+    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value]; });
+    __syncthreads();
+    sfor<R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int k = l + LQ * j;
+        double2 X = make_double2(0.0, 0.0);
+        if (k <= imlim) {
+            const double2 Zk = v[j];
+            const double2 Zm = seq[lds_slot(k == 0 ? 0 : M - k)];
+            const double2 A = make_double2(0.5 * (Zk.x + Zm.x), 0.5 * (Zk.y - Zm.y));
+            const double2 D = make_double2(Zk.x - Zm.x, Zk.y + Zm.y);
+            const double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
+            const double2 w = c_conj(rot96<(96 / (2 * R)) * j, 1>(wl));
+            X = c_add(A, c_mul(w, B));
+            if (k == 0) X.y = 0.0;
+        }
+        v[j] = X;
+    });
+    __syncthreads();
+    sfor<R>([&](auto jc) {
+        const int k = l + LQ * decltype(jc)::value;
+        if (k <= mmax) seq[lds_slot(k)] = v[decltype(jc)::value];
+    });
+    __syncthreads();
+    // 3) F out, latitude fastest
+    {
+        const int tl = threadIdx.x % T, row = threadIdx.x / T;
+        if (lat0 + tl < lat_end) {
+            double2 *Ff = F + (size_t)blockIdx.y * four_dist + lat0 + tl;
+            const double2 *src = lds + tl * STR;
+            sfor<R>([&](auto ic) {
+                const int k = row + (NT / T) * decltype(ic)::value;
+                if (k <= mmax) Ff[(size_t)k * nlat] = src[lds_slot(k)];
+            });
+        }
+    }
+}
+
 template <class K>
 void lds_limit(K kern, size_t lds) {
     ensure_lds_limit((const void *)kern, lds);
@@ -390,6 +533,67 @@ bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, in
                            long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
                            int lat_end) {
     return synth_wave(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
+}
+
+#define COOP_FFT_SIZES(X) X(32) X(48) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024) X(1536)
+
+bool fft_coop_supported(int M) {
+#define OK_CASE(MM) if (M == MM) return true;
+    COOP_FFT_SIZES(OK_CASE)
+#undef OK_CASE
+    return false;
+}
+
+template <int NTB>
+bool synth_coop(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F, long four_dist,
+                double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+#define SYN_COOP(MM)                                                                                          \
+    if (M == MM) {                                                                                            \
+        using CG = CoopGeom<MM, NTB>;                                                                         \
+        static_assert(CG::LQ <= 64 && CG::T >= 2 && CG::T * CG::LQ == CG::NT, "cooperative FFT shape");      \
+        lds_limit(fft_synth_coop<MM, NTB>, CG::lds_bytes);                                                    \
+        const int nb = (lat_end - lat_base + CG::T - 1) / CG::T;                                             \
+        hipLaunchKernelGGL((fft_synth_coop<MM, NTB>), dim3(nb, nfields), dim3(CG::NT), CG::lds_bytes, s, F,   \
+                           four_dist, V, spat_dist, t->roots, nlat, imlim, sp, st, lat_base, lat_end);        \
+        return true;                                                                                          \
+    }
+    COOP_FFT_SIZES(SYN_COOP)
+#undef SYN_COOP
+    return false;
+}
+
+template <int NTB>
+bool anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
+               long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+#define ANA_COOP(MM)                                                                                          \
+    if (M == MM) {                                                                                            \
+        using CG = CoopGeom<MM, NTB>;                                                                         \
+        lds_limit(fft_anal_coop<MM, NTB>, CG::lds_bytes);                                                     \
+        const int nb = (lat_end - lat_base + CG::T - 1) / CG::T;                                             \
+        hipLaunchKernelGGL((fft_anal_coop<MM, NTB>), dim3(nb, nfields), dim3(CG::NT), CG::lds_bytes, s, V,    \
+                           spat_dist, F, four_dist, t->roots, nlat, mmax, imlim, sp, st, lat_base, lat_end);  \
+        return true;                                                                                          \
+    }
+    COOP_FFT_SIZES(ANA_COOP)
+#undef ANA_COOP
+    return false;
+}
+
+bool launch_fft_synth_coop(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
+                           long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
+                           int lat_end, int nthreads) {
+    if (nthreads == 128)
+        return synth_coop<128>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
+    return synth_coop<256>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
+}
+
+bool launch_fft_anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
+                          long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
+                          int lat_end, int nthreads) {
+    if (nthreads == 128)
+        return anal_coop<128>(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base,
+                              lat_end);
+    return anal_coop<256>(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end);
 }
 bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
                            long four_dist, float *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,

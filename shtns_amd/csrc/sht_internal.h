@@ -132,7 +132,8 @@ void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dis
 // Spatial element (ip, it) of a field sits at ip*sp + it*st (theta-contiguous: sp=nlat, st=1).
 // lat_base / nlat_chunk restrict the transform to latitudes [lat_base, lat_base + nlat_chunk)
 // (nlat_chunk < 0: all latitudes).
-// kind: 0 automatic, 1 register FFT, 2 LDS-staged FFT (where a kernel for the length exists)
+// kind: 0 automatic, 1 register FFT, 2 LDS-staged FFT (where a kernel for the length exists),
+// 3 / 4 block-cooperative register FFT with 256 / 128 threads per block
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                       const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s,
                       int lat_base = 0, int nlat_chunk = -1, int kind = 0);
@@ -155,6 +156,15 @@ bool launch_fft_anal_wave(const DeviceTables *t, int nlat, int M, int mmax, int 
                           long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
                           int lat_end);
 bool fft_wave_supported(int M);
+// block-cooperative register FFT (sht_fft.hip): T = 256/LQ latitudes per block, HBM accesses
+// with the latitude fastest (theta-contiguous layout); M = nphi/2 in {32, 48, ..., 1536}
+bool fft_coop_supported(int M);
+bool launch_fft_synth_coop(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
+                           long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
+                           int lat_end, int nthreads = 256);
+bool launch_fft_anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
+                          long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
+                          int lat_end, int nthreads = 256);
 bool lds_fft_specialized(int M);   // an LDS-staged FFT kernel compiled for this length
 // raise a kernel's dynamic LDS limit once (outside any launch sequence that may be captured)
 void ensure_lds_limit(const void *kern, size_t lds);

@@ -490,17 +490,25 @@ bool lds_fft_specialized(int M) {
     return false;
 }
 
-// Register FFT (sht_fft.hip) or LDS-staged FFT: the register kernel keeps one latitude per
-// wave, so its spatial accesses are contiguous only in the phi-contiguous layout (sp == 1);
-// in the theta-contiguous layout the LDS kernel, whose blocks interleave two latitudes, is
-// faster where a specialized one exists.  SHTNS_AMD_FFT=wave|lds forces one of them.
-static bool use_wave_fft(int M, long sp, int kind) {
+// FFT kernel for a length, layout and direction: 1 register FFT, one latitude per 2..256
+// lanes (sht_fft.hip); 2 LDS-staged FFT (below); 3 / 4 block-cooperative register FFT with 256
+// / 128 threads (sht_fft.hip).  The register kernel keeps each latitude's accesses separate,
+// contiguous only in the phi-contiguous layout (sp == 1).  In the theta-contiguous layout the
+// cooperative kernel is fastest for synthesis (lmax=1023: 0.41 vs 0.55 ms) and the LDS kernel
+// for analysis (0.545 vs 0.60 ms).  kind (autotuner) or SHTNS_AMD_FFT=wave|lds|coop override.
+static int fft_choice(int M, long sp, int kind, bool synthesis) {
     static const char *env = getenv("SHTNS_AMD_FFT");
-    if (kind == 1) return true;
-    if (kind == 2 && lds_fft_specialized(M)) return false;
-    if (env && !strcmp(env, "lds")) return false;
-    if (env && !strcmp(env, "wave")) return true;
-    return sp == 1 || !lds_fft_specialized(M);
+    int want = kind;
+    if (!want && env)
+        want = !strcmp(env, "wave") ? 1 : !strcmp(env, "lds") ? 2 : !strcmp(env, "coop") ? 3 : !strcmp(env, "coop128") ? 4 : 0;
+    if (want == 2 && lds_fft_specialized(M)) return 2;
+    if ((want == 3 || want == 4) && fft_coop_supported(M)) return want;
+    if (want == 1 && fft_wave_supported(M)) return 1;
+    if (sp == 1 && fft_wave_supported(M)) return 1;
+    if (synthesis && fft_coop_supported(M)) return 3;
+    if (lds_fft_specialized(M)) return 2;
+    if (fft_coop_supported(M)) return 3;
+    return fft_wave_supported(M) ? 1 : 2;
 }
 
 // hipFuncSetAttribute once per kernel and size: kept out of the launch path so launches can
@@ -528,7 +536,12 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
     if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
-    if (!generic && use_wave_fft(M, sp, kind) &&
+    const int choice = generic ? 2 : fft_choice(M, sp, kind, true);
+    if ((choice == 3 || choice == 4) &&
+        launch_fft_synth_coop(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end,
+                              choice == 4 ? 128 : 256))
+        return;
+    if (choice == 1 &&
         launch_fft_synth_wave(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end))
         return;
 #define FFT_SYN_CASE(MM)                                                                                        \
@@ -558,7 +571,12 @@ void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int im
     if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
-    if (!generic && use_wave_fft(M, sp, kind) &&
+    const int choice = generic ? 2 : fft_choice(M, sp, kind, false);
+    if ((choice == 3 || choice == 4) &&
+        launch_fft_anal_coop(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end,
+                             choice == 4 ? 128 : 256))
+        return;
+    if (choice == 1 &&
         launch_fft_anal_wave(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end))
         return;
 #define FFT_ANA_CASE(MM)                                                                                          \
