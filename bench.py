@@ -107,7 +107,7 @@ def main():
                     help="pair: the headline scalar pair at lmax=1023 (default); uneven: 256 transforms "
                          "of uneven lmax from one graph plus one lmax=4095 field; vector: spheroidal/"
                          "toroidal pairs at lmax=511")
-    ap.add_argument("--tune", action="store_true", help="uneven workload: autotune each configuration first")
+    ap.add_argument("--tune", action="store_true", help="autotune the kernel shapes (cushtns_tune) before timing")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
@@ -137,6 +137,10 @@ def main():
     stream = torch.cuda.current_stream(dev)
     lib.cushtns_set_streams(s._cfg, ctypes.c_void_p(stream.cuda_stream), ctypes.c_void_p(stream.cuda_stream))
     qp, vp, q2p = (ctypes.c_void_p(t.data_ptr()) for t in (q, v, q2))
+    tuned = None
+    if args.tune:
+        from shtns_amd.batch import tune
+        tuned = tune(s, BATCH)
 
     def step():
         lib.cu_SH_to_spat_batch(s._cfg, qp, vp, LMAX, BATCH, nlm, nspat)
@@ -231,7 +235,8 @@ def main():
         "data": "synthetic (SplitMix64/Box-Muller a_lm ~ N(0,(l+1)^-2), seed 1+field)",
         "config": {"workload": "SH_to_spat+spat_to_SH, lmax=mmax=1023, Gauss nlat=1536 nphi=3072, "
                                f"{args.layout}-contiguous, batch 16 FP64 fields",
-                   "lmax": LMAX, "nlat": NLAT, "nphi": NPHI, "batch": BATCH, "parallelism": "single GPU"},
+                   "lmax": LMAX, "nlat": NLAT, "nphi": NPHI, "batch": BATCH, "parallelism": "single GPU",
+                   "tuned_shapes": tuned},
         "fp64_frac_of_peak_whole_pair": round(alg_total / elapsed / 1e12 / FP64_PEAK_TFLOPS, 4),
         "round_trip_rel_l2": err,
         "kernels": {k: {kk: round(vv, 4) if isinstance(vv, float) else vv for kk, vv in d.items()}

@@ -53,6 +53,16 @@ __device__ inline OrderInfo order_info(int im, int lmax_spec, int mmax, int mres
     return o;
 }
 
+// Workgroups are dealt round-robin to the 8 XCDs (block b runs on XCD b % 8).  Map physical
+// block b to a logical index so that each run of `gsize` consecutive logical blocks lands on
+// one XCD, in dispatch order: XCD x takes groups x, x + 8, x + 16, ...  Identity unless nb is
+// a multiple of 8 * gsize.
+__device__ inline int xcd_cluster(int b, int nb, int gsize) {
+    if (gsize <= 1 || nb % (8 * gsize)) return b;
+    const int x = b & 7, s = b >> 3;
+    return (x + 8 * (s / gsize)) * gsize + s % gsize;
+}
+
 // ---------------------------------------------------------------------------------------
 // Contraction precision PREC (the recurrence, seeds, tables and the final sums stay FP64):
 //   0  FP64 matrix cores: v_mfma_f64_16x16x4f64 on z and the coefficients;
@@ -136,10 +146,12 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    // heavy orders first: block x -> (order, tile group); all tile groups of an order adjacent
+    // heavy orders first: block x -> (order, tile group); the tile groups of one order run on
+    // one XCD (they read the same coefficient rows, which then hit in that XCD's L2)
     const int ntg = (ntl + kSynthWaves - 1) / kSynthWaves;   // tile groups of this launch
-    const int im = blockIdx.x / ntg;
-    const int tile = tile0 + (blockIdx.x - im * ntg) * kSynthWaves + wave;
+    const int bx = xcd_cluster(blockIdx.x, gridDim.x, ntg);
+    const int im = bx / ntg;
+    const int tile = tile0 + (bx - im * ntg) * kSynthWaves + wave;
     const int g = blockIdx.y;
     const int fbase = g * 4 * NCG;
     if (tile >= tile0 + ntl || tile * 64 >= nlat_2) return;   // no block-level synchronization in this kernel
@@ -418,6 +430,13 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     // post-mix work split: thread handles elements e = threadIdx.x + j*64*NW (j < NE) of the
     // 16 x NCOL block; their coefficient rows are prefetched before the MFMA phase.
     constexpr int NE = (16 * NCOL + 64 * NW - 1) / (64 * NW);
+    // recurrence coefficients of a 16-step group (uniform: scalar loads); the next group's are
+    // loaded after the MFMAs are issued, so they arrive during the reduction
+    double2 cf[16];
+    if (nsteps > 0) {
+#pragma unroll
+        for (int s = 0; s < 16; ++s) cf[s] = abm[min(ibeg + s, nsteps - 1)];
+    }
     int b = 0;
     for (int i = ibeg; i < nsteps; i += 16, ++b) {
         double pcx[NE], pcz[NE], pcp[NE];
@@ -431,9 +450,6 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             pcz[j] = cf4.z;
             pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
         }
-        double2 cf[16];
-#pragma unroll
-        for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
         typename C::acc_t acc[NCG], acc2[NCG];   // two chains per column group (even / odd K-groups)
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] = acc2[c] = C::zero();
@@ -473,6 +489,8 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
                     acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
                 }
         }
+#pragma unroll
+        for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + 16 + s, nsteps - 1)];
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] += acc2[c];
         // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
