@@ -74,10 +74,13 @@ int cushtns_set_precision(shtns_cfg, int prec);
  * cushtns_tuning reads the five choices (0 = automatic):
  * {synthesis groups, analysis waves, analysis groups, synthesis FFT, analysis FFT}, with FFT
  * 1 = register kernel, 2 = LDS-staged kernel, 3 / 4 = block-cooperative register kernel with
- * 256 / 128 threads per block.
+ * 256 / 128 threads per block, 5 / 6 = the same with 8-byte LDS staging, 256 / 512 threads.
  * Both return 0, or -1 on error. */
 int cushtns_tune(shtns_cfg, int nfields);
 int cushtns_tuning(shtns_cfg, int *choices);
+/* Set the five choices by hand (same meaning; a choice without a compiled kernel for this
+ * size falls back to the automatic one).  Returns 0, or -1 for a value out of range. */
+int cushtns_set_tuning(shtns_cfg, const int *choices);
 
 /* Uneven batches (additive, no SHTns counterpart): n scalar transforms, transform i on
  * configuration cfgs[i] (any mix of lmax / grids, grids already set).  The batch owns the

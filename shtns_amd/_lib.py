@@ -25,7 +25,7 @@ EXPORTED_SYMBOLS = [
     "cu_spat_to_SHsphtor_batch", "cu_SH_to_spat_adjoint_batch", "cu_spat_to_SH_adjoint_batch",
     "shtns_profile", "shtns_profile_get",
     "cu_spat_to_SH_float", "cu_SH_to_spat_float", "cu_SH_to_spat_float_batch", "cu_spat_to_SH_float_batch",
-    "cushtns_set_precision", "cushtns_tune", "cushtns_tuning", "cushtns_batch_create", "cushtns_batch_arrays",
+    "cushtns_set_precision", "cushtns_tune", "cushtns_tuning", "cushtns_set_tuning", "cushtns_batch_create", "cushtns_batch_arrays",
     "cushtns_batch_SH_to_spat", "cushtns_batch_spat_to_SH", "cushtns_batch_destroy",
 ]
 
@@ -123,6 +123,7 @@ def load():
         "cushtns_set_precision": (I, [CFG, I]),
         "cushtns_tune": (I, [CFG, I]),
         "cushtns_tuning": (I, [CFG, P]),
+        "cushtns_set_tuning": (I, [CFG, P]),
         "cushtns_batch_create": (P, [I, P, I, I]),
         "cushtns_batch_arrays": (I, [P, I, P, P]),
         "cushtns_batch_SH_to_spat": (I, [P, P]),

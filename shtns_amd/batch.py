@@ -38,6 +38,13 @@ def tune(sh, nfields):
     return tuple(out)
 
 
+def set_tuning(sh, choices):
+    """Set the five kernel-shape choices of configuration `sh` by hand (see tune)."""
+    arr = (ctypes.c_int * 5)(*[int(c) for c in choices])
+    if _lib.load().cushtns_set_tuning(sh._cfg, arr) != 0:
+        _lib.check_error("set_tuning")
+
+
 class UnevenBatch:
     def __init__(self, shts, prec=SHT_PREC_AUTO, tune=False):
         import torch

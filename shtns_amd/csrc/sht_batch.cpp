@@ -82,8 +82,7 @@ extern "C" int cushtns_tune(shtns_cfg c, int nfields) {
     if (fft_wave_supported(M)) fft_kinds.push_back(1);
     if (lds_fft_specialized(M)) fft_kinds.push_back(2);
     if (fft_coop_supported(M)) {
-        fft_kinds.push_back(3);
-        fft_kinds.push_back(4);
+        for (int k = 3; k <= 6; ++k) fft_kinds.push_back(k);
     }
     if (fft_kinds.size() < 2) fft_kinds = {0};
 
@@ -142,6 +141,27 @@ extern "C" int cushtns_tuning(shtns_cfg c, int *choices) {
     choices[2] = c->tune_anal_ncg;
     choices[3] = c->tune_fft_synth;
     choices[4] = c->tune_fft_anal;
+    return 0;
+}
+
+extern "C" int cushtns_set_tuning(shtns_cfg c, const int *choices) {
+    clear_error();
+    if (c == nullptr || choices == nullptr) {
+        set_error(1, "cushtns_set_tuning: null argument");
+        return -1;
+    }
+    const int ncg = choices[0], nw = choices[1], ancg = choices[2], fs = choices[3], fa = choices[4];
+    const bool ok = (ncg == 0 || ncg == 1 || ncg == 2 || ncg == 4) && nw >= 0 && nw <= 12 && ancg >= 0 &&
+                    ancg <= 2 && fs >= 0 && fs <= 6 && fa >= 0 && fa <= 6;
+    if (!ok) {
+        set_error(1, "cushtns_set_tuning: invalid choice");
+        return -1;
+    }
+    c->tune_synth_ncg = ncg;
+    c->tune_anal_nw = nw;
+    c->tune_anal_ncg = ancg;
+    c->tune_fft_synth = fs;
+    c->tune_fft_anal = fa;
     return 0;
 }
 

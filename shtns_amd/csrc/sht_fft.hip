@@ -228,6 +228,68 @@ __device__ __forceinline__ void wave_passes(double2 (&v)[WaveGeom<M>::R], double
     }
 }
 
+// The same passes with each exchange moved through an 8-byte-per-point LDS buffer: real
+// parts, then imaginary parts (half the LDS of wave_passes, two more barriers per exchange).
+template <int M, int SGN, int S>
+__device__ __forceinline__ void wave_passes_split(double2 (&v)[WaveGeom<M>::R], double *seq, int l,
+                                                  const double2 *__restrict__ roots) {
+    using G = WaveGeom<M>;
+    constexpr int R = G::R, LQ = G::LQ;
+    if constexpr (S < G::PL.n) {
+        constexpr int P = G::PL.r[S];
+        constexpr int p = plan_p(M, R, S);
+        constexpr int NB = R / P;
+        constexpr bool last = (S + 1 == G::PL.n);
+        constexpr int tw_step = 2 * M / (p * P);
+        // butterflies in place: outputs of butterfly b replace its inputs v[b + r NB]
+        sfor<NB>([&](auto bc) {
+            constexpr int b = decltype(bc)::value;
+            const int i = l + LQ * b;
+            const int k = (p == 1) ? 0 : i % p;
+            double2 u[P];
+            sfor<P>([&](auto rc) {
+                constexpr int r = decltype(rc)::value;
+                u[r] = v[b + r * NB];
+                if constexpr (p > 1 && r > 0) {
+                    double2 w = roots[r * k * tw_step];
+                    if constexpr (SGN < 0) w.y = -w.y;
+                    u[r] = c_mul(u[r], w);
+                }
+            });
+            dft<P, SGN>(u);
+            sfor<P>([&](auto rc) { v[b + decltype(rc)::value * NB] = u[decltype(rc)::value]; });
+        });
+        if constexpr (!last) {
+            // output r of butterfly b goes to point (i - k) P + k + r p
+            auto put = [&](auto part) {
+                sfor<NB>([&](auto bc) {
+                    constexpr int b = decltype(bc)::value;
+                    const int i = l + LQ * b;
+                    const int k = (p == 1) ? 0 : i % p;
+                    const int base = (i - k) * P + k;
+                    sfor<P>([&](auto rc) {
+                        constexpr int r = decltype(rc)::value;
+                        seq[lds_slot(base + r * p)] = decltype(part)::value ? v[b + r * NB].y : v[b + r * NB].x;
+                    });
+                });
+            };
+            double nx[R];
+            put(std::integral_constant<int, 0>{});
+            __syncthreads();
+            sfor<R>([&](auto jc) { nx[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+            __syncthreads();
+            put(std::integral_constant<int, 1>{});
+            __syncthreads();
+            sfor<R>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                v[j] = make_double2(nx[j], seq[lds_slot(l + LQ * j)]);
+            });
+            __syncthreads();
+            wave_passes_split<M, SGN, S + 1>(v, seq, l, roots);
+        }
+    }
+}
+
 // Blocks are dealt round-robin to the 8 XCDs; give each XCD a contiguous range of latitude
 // groups so that the blocks running together on one XCD touch neighbouring latitudes (which
 // share cache lines of that XCD's L2 in the theta-contiguous layout).
@@ -475,6 +537,185 @@ __global__ __launch_bounds__(NTB) void fft_anal_coop(const double *__restrict__ 
     }
 }
 
+// Cooperative kernels with 8-byte LDS staging (real and imaginary parts in turn): T
+// latitudes in half the LDS of fft_*_coop (256 threads, M = 1536: T = 4 in 52 KB, three
+// blocks per CU).  HBM accesses keep their full width (16-byte F, 8-byte V elements).
+template <int M, int NTB>
+struct CoopSGeom {
+    using G = WaveGeom<M>;
+    static constexpr int LQ = G::LQ, R = G::R;
+    static constexpr int NT = NTB;
+    static constexpr int T = NT / LQ;
+    static constexpr int STR = lds_seq_stride(M);   // doubles per sequence buffer
+    static constexpr size_t lds_bytes = (size_t)T * STR * sizeof(double);
+};
+
+template <int M, int NTB>
+__global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict__ F, long four_dist,
+                                                       double *__restrict__ V, long spat_dist,
+                                                       const double2 *__restrict__ roots, int nlat, int imlim,
+                                                       long sp, long stt, int lat_base, int lat_end) {
+    using CG = CoopSGeom<M, NTB>;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = CG::STR, ROWS = NT / T;
+    extern __shared__ double ldsd[];
+    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
+    const int tl = threadIdx.x % T, row = threadIdx.x / T;
+    const bool live = lat0 + tl < lat_end;
+    double *mine = ldsd + tl * STR;                // staging rows of latitude tl
+    const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
+    double *seq = ldsd + sq * STR;                 // sequence sq
+    // X_k of the block's latitudes, 16-byte loads with the latitude fastest
+    double2 xr[R];
+    {
+        const double2 *Ff = F + (size_t)blockIdx.y * four_dist + (live ? lat0 + tl : 0);
+        sfor<R>([&](auto ic) {
+            const int k = row + ROWS * decltype(ic)::value;
+            xr[decltype(ic)::value] = (live && k <= imlim) ? Ff[(size_t)k * nlat] : make_double2(0.0, 0.0);
+        });
+    }
+    // real parts: keep s = Re X_k + Re X_{M-k}, d = Re X_k - Re X_{M-k}
+    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = xr[decltype(ic)::value].x; });
+    __syncthreads();
+    double sr[R], dr[R];
+    sfor<R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int k = l + LQ * j, km = M - k;
+        const double a = (k <= imlim) ? seq[lds_slot(k)] : 0.0;
+        const double b = (k > 0 && km <= imlim) ? seq[lds_slot(km)] : 0.0;
+        sr[j] = a + b;
+        dr[j] = a - b;
+    });
+    __syncthreads();
+    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = xr[decltype(ic)::value].y; });
+    __syncthreads();
+    // Z_k = A + i w B, A = X_k + conj X_{M-k}, B = X_k - conj X_{M-k}, w = e^{i pi k/M}
+    const double2 wl = roots[l];
+    double2 v[R];
+    sfor<R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int k = l + LQ * j, km = M - k;
+        const double a = (k <= imlim) ? seq[lds_slot(k)] : 0.0;
+        const double b = (k > 0 && km <= imlim) ? seq[lds_slot(km)] : 0.0;
+        const double2 A = make_double2(sr[j], a - b), B = make_double2(dr[j], a + b);
+        const double2 w = rot96<(96 / (2 * R)) * j, 1>(wl);
+        const double2 wB = c_mul(w, B);
+        v[j] = make_double2(A.x - wB.y, A.y + wB.x);
+    });
+    __syncthreads();
+    wave_passes_split<M, 1, 0>(v, seq, l, roots);
+    // z_n out: real parts to rows 2n, then imaginary parts to rows 2n+1
+    double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(live ? lat0 + tl : 0) * stt;
+    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].x; });
+    __syncthreads();
+    if (live)
+        sfor<R>([&](auto ic) {
+            const long n = row + ROWS * decltype(ic)::value;
+            Vf[(2 * n) * sp] = mine[lds_slot((int)n)];
+        });
+    __syncthreads();
+    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].y; });
+    __syncthreads();
+    if (live)
+        sfor<R>([&](auto ic) {
+            const long n = row + ROWS * decltype(ic)::value;
+            Vf[(2 * n + 1) * sp] = mine[lds_slot((int)n)];
+        });
+}
+
+template <int M, int NTB>
+__global__ __launch_bounds__(NTB) void fft_anal_coops(const double *__restrict__ V, long spat_dist,
+                                                      double2 *__restrict__ F, long four_dist,
+                                                      const double2 *__restrict__ roots, int nlat, int mmax,
+                                                      int imlim, long sp, long stt, int lat_base, int lat_end) {
+    using CG = CoopSGeom<M, NTB>;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = CG::STR, ROWS = NT / T;
+    extern __shared__ double ldsd[];
+    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
+    const int tl = threadIdx.x % T, row = threadIdx.x / T;
+    const bool live = lat0 + tl < lat_end;
+    double *mine = ldsd + tl * STR;
+    const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
+    double *seq = ldsd + sq * STR;
+    // z_n = V[2n] + i V[2n+1] of the block's latitudes, latitude fastest
+    double zx[R], zy[R];
+    {
+        const double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(live ? lat0 + tl : 0) * stt;
+        sfor<R>([&](auto ic) {
+            const long n = row + ROWS * decltype(ic)::value;
+            zx[decltype(ic)::value] = live ? Vf[(2 * n) * sp] : 0.0;
+            zy[decltype(ic)::value] = live ? Vf[(2 * n + 1) * sp] : 0.0;
+        });
+    }
+    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = zx[decltype(ic)::value]; });
+    __syncthreads();
+    double vx[R];
+    sfor<R>([&](auto jc) { vx[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+    __syncthreads();
+    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = zy[decltype(ic)::value]; });
+    __syncthreads();
+    double2 v[R];
+    sfor<R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        v[j] = make_double2(vx[j], seq[lds_slot(l + LQ * j)]);
+    });
+    __syncthreads();
+    const double2 wl = roots[l];
+    wave_passes_split<M, -1, 0>(v, seq, l, roots);
+    // X_k = A + conj(w_k) B with A = (Z_k + conj Z_{M-k})/2, B = (Z_k - conj Z_{M-k})/(2i):
+    // Z_{M-k} from LDS, real then imaginary parts
+    double zmx[R];
+    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].x; });
+    __syncthreads();
+    sfor<R>([&](auto jc) {
+        const int k = l + LQ * decltype(jc)::value;
+        zmx[decltype(jc)::value] = seq[lds_slot(k == 0 ? 0 : M - k)];
+    });
+    __syncthreads();
+    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].y; });
+    __syncthreads();
+    sfor<R>([&](auto jc) {
+        constexpr int j = decltype(jc)::value;
+        const int k = l + LQ * j;
+        const double2 Zm = make_double2(zmx[j], seq[lds_slot(k == 0 ? 0 : M - k)]);
+        double2 X = make_double2(0.0, 0.0);
+        if (k <= imlim) {
+            const double2 Zk = v[j];
+            const double2 A = make_double2(0.5 * (Zk.x + Zm.x), 0.5 * (Zk.y - Zm.y));
+            const double2 D = make_double2(Zk.x - Zm.x, Zk.y + Zm.y);
+            const double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
+            const double2 w = c_conj(rot96<(96 / (2 * R)) * j, 1>(wl));
+            X = c_add(A, c_mul(w, B));
+            if (k == 0) X.y = 0.0;
+        }
+        v[j] = X;
+    });
+    __syncthreads();
+    // F out with 16-byte stores, latitude fastest: real parts through LDS, then imaginary parts
+    double fx[R];
+    sfor<R>([&](auto jc) {
+        const int k = l + LQ * decltype(jc)::value;
+        if (k <= mmax) seq[lds_slot(k)] = v[decltype(jc)::value].x;
+    });
+    __syncthreads();
+    sfor<R>([&](auto ic) {
+        const int k = row + ROWS * decltype(ic)::value;
+        fx[decltype(ic)::value] = (k <= mmax) ? mine[lds_slot(k)] : 0.0;
+    });
+    __syncthreads();
+    sfor<R>([&](auto jc) {
+        const int k = l + LQ * decltype(jc)::value;
+        if (k <= mmax) seq[lds_slot(k)] = v[decltype(jc)::value].y;
+    });
+    __syncthreads();
+    if (live) {
+        double2 *Ff = F + (size_t)blockIdx.y * four_dist + lat0 + tl;
+        sfor<R>([&](auto ic) {
+            const int k = row + ROWS * decltype(ic)::value;
+            if (k <= mmax) Ff[(size_t)k * nlat] = make_double2(fx[decltype(ic)::value], mine[lds_slot(k)]);
+        });
+    }
+}
+
 template <class K>
 void lds_limit(K kern, size_t lds) {
     ensure_lds_limit((const void *)kern, lds);
@@ -579,9 +820,48 @@ bool anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int 
     return false;
 }
 
+template <int NTB>
+bool synth_coops(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F, long four_dist,
+                 double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+#define SYN_COOPS(MM)                                                                                         \
+    if (M == MM) {                                                                                            \
+        using CG = CoopSGeom<MM, NTB>;                                                                        \
+        static_assert(CG::LQ <= 64 && CG::T >= 2 && CG::T * CG::LQ == CG::NT, "cooperative FFT shape");      \
+        lds_limit(fft_synth_coops<MM, NTB>, CG::lds_bytes);                                                   \
+        const int nb = (lat_end - lat_base + CG::T - 1) / CG::T;                                             \
+        hipLaunchKernelGGL((fft_synth_coops<MM, NTB>), dim3(nb, nfields), dim3(CG::NT), CG::lds_bytes, s, F,  \
+                           four_dist, V, spat_dist, t->roots, nlat, imlim, sp, st, lat_base, lat_end);        \
+        return true;                                                                                          \
+    }
+    COOP_FFT_SIZES(SYN_COOPS)
+#undef SYN_COOPS
+    return false;
+}
+
+template <int NTB>
+bool anal_coops(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
+                long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+#define ANA_COOPS(MM)                                                                                         \
+    if (M == MM) {                                                                                            \
+        using CG = CoopSGeom<MM, NTB>;                                                                        \
+        lds_limit(fft_anal_coops<MM, NTB>, CG::lds_bytes);                                                    \
+        const int nb = (lat_end - lat_base + CG::T - 1) / CG::T;                                             \
+        hipLaunchKernelGGL((fft_anal_coops<MM, NTB>), dim3(nb, nfields), dim3(CG::NT), CG::lds_bytes, s, V,   \
+                           spat_dist, F, four_dist, t->roots, nlat, mmax, imlim, sp, st, lat_base, lat_end);  \
+        return true;                                                                                          \
+    }
+    COOP_FFT_SIZES(ANA_COOPS)
+#undef ANA_COOPS
+    return false;
+}
+
 bool launch_fft_synth_coop(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
                            long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
                            int lat_end, int nthreads) {
+    if (nthreads == -256)
+        return synth_coops<256>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
+    if (nthreads == -512)
+        return synth_coops<512>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
     if (nthreads == 128)
         return synth_coop<128>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
     return synth_coop<256>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
@@ -590,6 +870,12 @@ bool launch_fft_synth_coop(const DeviceTables *t, int nlat, int M, int imlim, in
 bool launch_fft_anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
                           long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
                           int lat_end, int nthreads) {
+    if (nthreads == -256)
+        return anal_coops<256>(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base,
+                               lat_end);
+    if (nthreads == -512)
+        return anal_coops<512>(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base,
+                               lat_end);
     if (nthreads == 128)
         return anal_coop<128>(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base,
                               lat_end);

@@ -133,7 +133,8 @@ void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dis
 // lat_base / nlat_chunk restrict the transform to latitudes [lat_base, lat_base + nlat_chunk)
 // (nlat_chunk < 0: all latitudes).
 // kind: 0 automatic, 1 register FFT, 2 LDS-staged FFT (where a kernel for the length exists),
-// 3 / 4 block-cooperative register FFT with 256 / 128 threads per block
+// 3 / 4 block-cooperative register FFT with 256 / 128 threads per block, 5 / 6 the same with
+// 8-byte LDS staging and 256 / 512 threads
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                       const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s,
                       int lat_base = 0, int nlat_chunk = -1, int kind = 0);

@@ -492,22 +492,32 @@ bool lds_fft_specialized(int M) {
 
 // FFT kernel for a length, layout and direction: 1 register FFT, one latitude per 2..256
 // lanes (sht_fft.hip); 2 LDS-staged FFT (below); 3 / 4 block-cooperative register FFT with 256
-// / 128 threads (sht_fft.hip).  The register kernel keeps each latitude's accesses separate,
-// contiguous only in the phi-contiguous layout (sp == 1).  In the theta-contiguous layout the
-// cooperative kernel is fastest for synthesis (lmax=1023: 0.41 vs 0.55 ms) and the LDS kernel
-// for analysis (0.545 vs 0.60 ms).  kind (autotuner) or SHTNS_AMD_FFT=wave|lds|coop override.
+// / 128 threads, 5 / 6 the same with 8-byte LDS staging and 256 / 512 threads (sht_fft.hip).
+// The register kernel keeps each latitude's accesses separate, contiguous only in the
+// phi-contiguous layout (sp == 1).  In the theta-contiguous layout, measured at lmax=1023 with
+// 16 fields: synthesis 0.41 ms with kind 3 (LDS kernel 0.55), analysis 0.48 ms with kind 6
+// (8 latitudes per block, 64-byte V runs; LDS kernel 0.57).  kind (autotuner) or
+// SHTNS_AMD_FFT=wave|lds|coop|coop128|coops|coops512 override.
+// threads per block of the cooperative kinds (negative: 8-byte LDS staging)
+static int coop_threads(int kind) { return kind == 4 ? 128 : kind == 5 ? -256 : kind == 6 ? -512 : 256; }
+
 static int fft_choice(int M, long sp, int kind, bool synthesis) {
     static const char *env = getenv("SHTNS_AMD_FFT");
     int want = kind;
     if (!want && env)
-        want = !strcmp(env, "wave") ? 1 : !strcmp(env, "lds") ? 2 : !strcmp(env, "coop") ? 3 : !strcmp(env, "coop128") ? 4 : 0;
+        want = !strcmp(env, "wave")       ? 1
+               : !strcmp(env, "lds")      ? 2
+               : !strcmp(env, "coop")     ? 3
+               : !strcmp(env, "coop128")  ? 4
+               : !strcmp(env, "coops")    ? 5
+               : !strcmp(env, "coops512") ? 6
+                                          : 0;
     if (want == 2 && lds_fft_specialized(M)) return 2;
-    if ((want == 3 || want == 4) && fft_coop_supported(M)) return want;
+    if (want >= 3 && want <= 6 && fft_coop_supported(M)) return want;
     if (want == 1 && fft_wave_supported(M)) return 1;
     if (sp == 1 && fft_wave_supported(M)) return 1;
-    if (synthesis && fft_coop_supported(M)) return 3;
+    if (fft_coop_supported(M)) return synthesis ? 3 : 6;
     if (lds_fft_specialized(M)) return 2;
-    if (fft_coop_supported(M)) return 3;
     return fft_wave_supported(M) ? 1 : 2;
 }
 
@@ -537,9 +547,9 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
     const int choice = generic ? 2 : fft_choice(M, sp, kind, true);
-    if ((choice == 3 || choice == 4) &&
+    if (choice >= 3 && choice <= 6 &&
         launch_fft_synth_coop(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end,
-                              choice == 4 ? 128 : 256))
+                              coop_threads(choice)))
         return;
     if (choice == 1 &&
         launch_fft_synth_wave(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end))
@@ -572,9 +582,9 @@ void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int im
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
     const int choice = generic ? 2 : fft_choice(M, sp, kind, false);
-    if ((choice == 3 || choice == 4) &&
+    if (choice >= 3 && choice <= 6 &&
         launch_fft_anal_coop(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end,
-                             choice == 4 ? 128 : 256))
+                             coop_threads(choice)))
         return;
     if (choice == 1 &&
         launch_fft_anal_wave(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end))
