@@ -382,26 +382,27 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
         my_ibeg = min(my_ibeg, i0);
         my_imax = max(my_imax, kv ? i0 : 0);
         // B operand for K-group kg: latitude tile_base + 4*kg + lane/16, column lane%16 of group c
+        // (E.re, E.im, O.re, O.im = weighted N+S, N+S, (N-S) x, (N-S) x).  Branch-free, every
+        // load issued before the first use: the block's first MFMA waits for these.
+        const int col = lane & 15, comp = col & 3;
+        const bool keep = nsteps > 0 && !(o.real_only && (comp & 1));
 #pragma unroll
         for (int c = 0; c < NCG; ++c) {
-            const int col = lane & 15;
             const int f = fbase + 4 * c + (col >> 2);
-            const int comp = col & 3;
+            const bool fv = keep && f < nfields;
+            const double2 *Ff = F + (size_t)(fv ? f : 0) * four_dist + (size_t)im * nlat;
 #pragma unroll
             for (int kg = 0; kg < 16; ++kg) {
                 const int kl = (wave + NW * r) * 64 + 4 * kg + (lane >> 4);
-                double v = 0.0;
-                if (kl < nlat_2 && f < nfields && nsteps > 0 && !(o.real_only && (comp & 1))) {
-                    const double2 *Ff = F + (size_t)f * four_dist + (size_t)im * nlat;
-                    const double2 fn = Ff[kl];
-                    const double2 fs = Ff[nlat - 1 - kl];
-                    const double w = lat_weight ? wqk[kl] : 1.0;
-                    if (comp == 0) v = (fn.x + fs.x) * w;
-                    else if (comp == 1) v = (fn.y + fs.y) * w;
-                    else if (comp == 2) v = (fn.x - fs.x) * w * xk[kl];
-                    else v = (fn.y - fs.y) * w * xk[kl];
-                }
-                Ereg[r][c][kg] = C::split(v);
+                const bool kv = fv && kl < nlat_2;
+                const int klc = kv ? kl : 0;
+                const double2 fn = Ff[klc];
+                const double2 fs = Ff[nlat - 1 - klc];
+                const double w = lat_weight ? wqk[klc] : 1.0;
+                const double x = xk[klc];
+                const double v = comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
+                               : comp == 2 ? (fn.x - fs.x) * x : (fn.y - fs.y) * x;
+                Ereg[r][c][kg] = C::split(kv ? v * w : 0.0);
             }
         }
     }
@@ -430,13 +431,6 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     // post-mix work split: thread handles elements e = threadIdx.x + j*64*NW (j < NE) of the
     // 16 x NCOL block; their coefficient rows are prefetched before the MFMA phase.
     constexpr int NE = (16 * NCOL + 64 * NW - 1) / (64 * NW);
-    // recurrence coefficients of a 16-step group (uniform: scalar loads); the next group's are
-    // loaded after the MFMAs are issued, so they arrive during the reduction
-    double2 cf[16];
-    if (nsteps > 0) {
-#pragma unroll
-        for (int s = 0; s < 16; ++s) cf[s] = abm[min(ibeg + s, nsteps - 1)];
-    }
     int b = 0;
     for (int i = ibeg; i < nsteps; i += 16, ++b) {
         double pcx[NE], pcz[NE], pcp[NE];
@@ -450,6 +444,9 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             pcz[j] = cf4.z;
             pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
         }
+        double2 cf[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
         typename C::acc_t acc[NCG], acc2[NCG];   // two chains per column group (even / odd K-groups)
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] = acc2[c] = C::zero();
@@ -489,8 +486,6 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
                     acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
                 }
         }
-#pragma unroll
-        for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + 16 + s, nsteps - 1)];
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] += acc2[c];
         // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
