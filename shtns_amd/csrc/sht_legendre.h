@@ -339,9 +339,13 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const double4 *__restrict__ pm, const int *__restrict__ i0tab, const double2 *__restrict__ z0tab,
     const double2 *__restrict__ F, long four_dist, double2 *__restrict__ Q, long spec_dist, int lmax_spec,
     int spec_ltop, int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
-    double mpos_scale, int lat_weight) {
+    double mpos_scale, int lat_weight, int cf_lds) {
     constexpr int NCOL = 16 * NCG;
     __shared__ double ytile[NW][16][65];
+    // cf_lds: the order's recurrence coefficients (alpha_i, beta_i), i < nsteps, are copied to
+    // LDS once per block (dynamic LDS), so each 16-step group reads them at LDS latency instead
+    // of waiting on scalar loads that miss to HBM
+    extern __shared__ double2 cfl[];
     __shared__ double red[NW][16][NCOL + 1];   // single-buffered: two barriers per block, 2 blocks/CU fit
     __shared__ double carry[2][NCOL];
     __shared__ int s_ibeg[NW];
@@ -395,14 +399,27 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             for (int kg = 0; kg < 16; ++kg) {
                 const int kl = (wave + NW * r) * 64 + 4 * kg + (lane >> 4);
                 const bool kv = fv && kl < nlat_2;
-                const int klc = kv ? kl : 0;
-                const double2 fn = Ff[klc];
-                const double2 fs = Ff[nlat - 1 - klc];
-                const double w = lat_weight ? wqk[klc] : 1.0;
-                const double x = xk[klc];
-                const double v = comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
-                               : comp == 2 ? (fn.x - fs.x) * x : (fn.y - fs.y) * x;
-                Ereg[r][c][kg] = C::split(kv ? v * w : 0.0);
+                if constexpr (NLR >= 3) {
+                    // several tiles per wave: one K-group at a time (loads hoisted together
+                    // would need more registers than the tiles' operands leave)
+                    double v = 0.0;
+                    if (kv) {
+                        const double2 fn = Ff[kl], fs = Ff[nlat - 1 - kl];
+                        const double w = lat_weight ? wqk[kl] : 1.0;
+                        v = (comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
+                             : comp == 2 ? (fn.x - fs.x) * xk[kl] : (fn.y - fs.y) * xk[kl]) * w;
+                    }
+                    Ereg[r][c][kg] = C::split(v);
+                } else {
+                    const int klc = kv ? kl : 0;
+                    const double2 fn = Ff[klc];
+                    const double2 fs = Ff[nlat - 1 - klc];
+                    const double w = lat_weight ? wqk[klc] : 1.0;
+                    const double x = xk[klc];
+                    const double v = comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
+                                   : comp == 2 ? (fn.x - fs.x) * x : (fn.y - fs.y) * x;
+                    Ereg[r][c][kg] = C::split(kv ? v * w : 0.0);
+                }
             }
         }
     }
@@ -413,6 +430,10 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     my_imax = __builtin_amdgcn_readfirstlane(my_imax);
     if (lane == 0) s_ibeg[wave] = my_ibeg;
     if (threadIdx.x < 2 * NCOL) (&carry[0][0])[threadIdx.x] = 0.0;
+    // (multi-tile waves keep their registers for the operands: scalar loads there)
+    if constexpr (NLR >= 3) cf_lds = 0;
+    if (cf_lds)
+        for (int e = threadIdx.x; e < nsteps; e += 64 * NW) cfl[e] = ab[sofs + e];
     __syncthreads();
     int ibeg = nsteps;
     for (int w = 0; w < NW; ++w) ibeg = min(ibeg, s_ibeg[w]);
@@ -445,8 +466,13 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
         }
         double2 cf[16];
+        if (cf_lds) {
 #pragma unroll
-        for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
+            for (int s = 0; s < 16; ++s) cf[s] = cfl[min(i + s, nsteps - 1)];   // uniform: LDS broadcast
+        } else {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
+        }
         typename C::acc_t acc[NCG], acc2[NCG];   // two chains per column group (even / odd K-groups)
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] = acc2[c] = C::zero();
@@ -553,10 +579,17 @@ void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, doub
                  int spec_ltop, hipStream_t s) {
     const DeviceTables *t = a.t;
     dim3 grid(a.mmax + 1, (a.nfields + 4 * NCG - 1) / (4 * NCG));
-    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC>), grid, dim3(64 * NW), 0, s, t->x, t->x2, t->st, t->wq,
-                       t->step_ofs, t->ab, t->pm, t->i0, t->z0, F, four_dist, Q, spec_dist,
+    // static LDS of the kernel, and the coefficient table if it fits beside it in 160 KB
+    constexpr size_t NCOL = 16 * NCG;
+    constexpr size_t static_lds = sizeof(double) * (NW * 16 * 65 + NW * 16 * (NCOL + 1) + 2 * NCOL) + sizeof(int) * NW;
+    const size_t cf_bytes = sizeof(double2) * (size_t)(a.ltr / 2 + 2);
+    const int cf_lds = (static_lds + cf_bytes + 1024 <= 160 * 1024) ? 1 : 0;
+    const size_t dyn = cf_lds ? cf_bytes : 0;
+    ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC>, dyn);
+    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC>), grid, dim3(64 * NW), dyn, s, t->x, t->x2, t->st,
+                       t->wq, t->step_ofs, t->ab, t->pm, t->i0, t->z0, F, four_dist, Q, spec_dist,
                        spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
-                       a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale, a.lat_weight);
+                       a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale, a.lat_weight, cf_lds);
 }
 
 template <int PREC>
