@@ -11,6 +11,7 @@
 // through LDS only between passes (M = 1536: radices 24, 8, 8 -> two LDS exchanges).  The
 // first pass reads HBM directly and the last pass leaves its outputs in the lane that
 // stores them, so nothing else is staged.
+#include <algorithm>
 #include <utility>
 
 #include "sht_internal.h"
@@ -170,6 +171,25 @@ constexpr int plan_p(int M, int R, int S) {   // product of the radices of passe
 // LDS slot of point j of a sequence: one spare slot every 16 points against bank conflicts
 __device__ __forceinline__ int lds_slot(int j) { return j + (j >> 4); }
 constexpr int lds_seq_stride(int M) { return M + M / 16 + 1; }
+// lds_slot(j0 + STEP i) for j0 + STEP i >= 0.  When 16 divides STEP the spare slots move in
+// step with i, the slot is linear in i and a compile-time i folds into the LDS instruction's
+// offset (one address register for all of a lane's points).
+template <int STEP>
+__device__ __forceinline__ int slot_at(int j0, int i) {
+    if constexpr (STEP % 16 == 0) return lds_slot(j0) + i * (STEP + STEP / 16);
+    else return lds_slot(j0 + STEP * i);
+}
+
+// Element at byte offset `off` from a block-uniform base: the access then uses the
+// scalar-base + 32-bit-offset address form (one VGPR per access, no 64-bit arithmetic).
+template <class E>
+__device__ __forceinline__ E &at(E *base, unsigned off) {
+    return *(E *)((char *)base + off);
+}
+template <class E>
+__device__ __forceinline__ const E &at(const E *base, unsigned off) {
+    return *(const E *)((const char *)base + off);
+}
 
 template <int M>
 struct WaveGeom {
@@ -180,6 +200,27 @@ struct WaveGeom {
     static constexpr WPlan PL = wave_plan(M, R);
     static constexpr size_t lds_bytes = (size_t)SPB * lds_seq_stride(M) * sizeof(double2);
 };
+
+// Twiddles w^r = exp(SGN 2 pi i r e / (2M)), 0 < r < P, of a butterfly with base exponent e
+// (roots[e] = exp(2 pi i e / (2M))): the power-of-two r are read from the table, the others
+// are products of two earlier ones (r = h + (r - h), h the highest power of two in r), at
+// most two roundings from the table values.  Fewer loads in flight keeps the passes'
+// register peak down.
+template <int P, int SGN>
+__device__ __forceinline__ void twiddles(double2 (&w)[P], const double2 *__restrict__ roots, int e) {
+    sfor<P>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        if constexpr (r > 0) {
+            constexpr int h = 1 << (31 - __builtin_clz(r));
+            if constexpr (h == r) {
+                w[r] = roots[r * e];
+                if constexpr (SGN < 0) w[r].y = -w[r].y;
+            } else {
+                w[r] = c_mul(w[h], w[r - h]);
+            }
+        }
+    });
+}
 
 // Passes S.. of the Stockham FFT on the lane's points v[j] = x[l + LQ j].  Pass S (radix P,
 // p = product of the earlier radices) runs butterflies i = l + LQ b, b < R/P, on inputs
@@ -201,27 +242,24 @@ __device__ __forceinline__ void wave_passes(double2 (&v)[WaveGeom<M>::R], double
             constexpr int b = decltype(bc)::value;
             const int i = l + LQ * b;
             const int k = (p == 1) ? 0 : i % p;
-            double2 u[P];
+            double2 u[P], w[P];
+            if constexpr (p > 1) twiddles<P, SGN>(w, roots, k * tw_step);
             sfor<P>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
                 u[r] = v[b + r * NB];
-                if constexpr (p > 1 && r > 0) {
-                    double2 w = roots[r * k * tw_step];
-                    if constexpr (SGN < 0) w.y = -w.y;
-                    u[r] = c_mul(u[r], w);
-                }
+                if constexpr (p > 1 && r > 0) u[r] = c_mul(u[r], w[r]);
             });
             dft<P, SGN>(u);
             if constexpr (last) {
                 sfor<P>([&](auto rc) { v[b + decltype(rc)::value * NB] = u[decltype(rc)::value]; });
             } else {
                 const int base = (i - k) * P + k;
-                sfor<P>([&](auto rc) { seq[lds_slot(base + decltype(rc)::value * p)] = u[decltype(rc)::value]; });
+                sfor<P>([&](auto rc) { seq[slot_at<p>(base, decltype(rc)::value)] = u[decltype(rc)::value]; });
             }
         });
         if constexpr (!last) {
             __syncthreads();
-            sfor<R>([&](auto jc) { v[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+            sfor<R>([&](auto jc) { v[decltype(jc)::value] = seq[slot_at<LQ>(l, decltype(jc)::value)]; });
             __syncthreads();
             wave_passes<M, SGN, S + 1>(v, seq, l, roots);
         }
@@ -246,15 +284,12 @@ __device__ __forceinline__ void wave_passes_split(double2 (&v)[WaveGeom<M>::R], 
             constexpr int b = decltype(bc)::value;
             const int i = l + LQ * b;
             const int k = (p == 1) ? 0 : i % p;
-            double2 u[P];
+            double2 u[P], w[P];
+            if constexpr (p > 1) twiddles<P, SGN>(w, roots, k * tw_step);
             sfor<P>([&](auto rc) {
                 constexpr int r = decltype(rc)::value;
                 u[r] = v[b + r * NB];
-                if constexpr (p > 1 && r > 0) {
-                    double2 w = roots[r * k * tw_step];
-                    if constexpr (SGN < 0) w.y = -w.y;
-                    u[r] = c_mul(u[r], w);
-                }
+                if constexpr (p > 1 && r > 0) u[r] = c_mul(u[r], w[r]);
             });
             dft<P, SGN>(u);
             sfor<P>([&](auto rc) { v[b + decltype(rc)::value * NB] = u[decltype(rc)::value]; });
@@ -269,20 +304,20 @@ __device__ __forceinline__ void wave_passes_split(double2 (&v)[WaveGeom<M>::R], 
                     const int base = (i - k) * P + k;
                     sfor<P>([&](auto rc) {
                         constexpr int r = decltype(rc)::value;
-                        seq[lds_slot(base + r * p)] = decltype(part)::value ? v[b + r * NB].y : v[b + r * NB].x;
+                        seq[slot_at<p>(base, r)] = decltype(part)::value ? v[b + r * NB].y : v[b + r * NB].x;
                     });
                 });
             };
             double nx[R];
             put(std::integral_constant<int, 0>{});
             __syncthreads();
-            sfor<R>([&](auto jc) { nx[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+            sfor<R>([&](auto jc) { nx[decltype(jc)::value] = seq[slot_at<LQ>(l, decltype(jc)::value)]; });
             __syncthreads();
             put(std::integral_constant<int, 1>{});
             __syncthreads();
             sfor<R>([&](auto jc) {
                 constexpr int j = decltype(jc)::value;
-                v[j] = make_double2(nx[j], seq[lds_slot(l + LQ * j)]);
+                v[j] = make_double2(nx[j], seq[slot_at<LQ>(l, j)]);
             });
             __syncthreads();
             wave_passes_split<M, SGN, S + 1>(v, seq, l, roots);
@@ -369,7 +404,7 @@ __global__ __launch_bounds__(WaveGeom<M>::NT) void fft_anal_wave(const TV *__res
     const double2 wl = roots[l];
     wave_passes<M, -1, 0>(v, seq, l, roots);
     // Z_{M-k} sits in another lane: exchange through LDS
-    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value]; });
+    sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value]; });
     __syncthreads();
     if (live) {
         double2 *Ff = F + (size_t)blockIdx.y * four_dist + lat;
@@ -380,7 +415,7 @@ __global__ __launch_bounds__(WaveGeom<M>::NT) void fft_anal_wave(const TV *__res
                 double2 X = make_double2(0.0, 0.0);
                 if (k <= imlim) {
                     const double2 Zk = v[j];
-                    const double2 Zm = seq[lds_slot(k == 0 ? 0 : M - k)];
+                    const double2 Zm = seq[(j == 0 && l == 0) ? 0 : slot_at<-LQ>(M - l, j)];   // Z_{(M-k) mod M}
                     const double2 A = make_double2(0.5 * (Zk.x + Zm.x), 0.5 * (Zk.y - Zm.y));
                     const double2 D = make_double2(Zk.x - Zm.x, Zk.y + Zm.y);
                     const double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
@@ -397,52 +432,75 @@ __global__ __launch_bounds__(WaveGeom<M>::NT) void fft_anal_wave(const TV *__res
 // ---- block-cooperative variant for the theta-contiguous layout ---------------------------
 // The register kernels above give each latitude its own lanes, so in the theta-contiguous
 // layout (consecutive latitudes adjacent in memory) every lane's load or store touches a
-// different cache line.  Here a block of 256 threads holds T = 256/LQ consecutive latitudes
+// different cache line.  Here a block of NT threads holds T = NT/LQ consecutive latitudes
 // and moves data between HBM and its LDS sequence buffers with the latitude index fastest
 // across threads (runs of T*16 B of F and T*8 B of V per cache line), then each sequence's
-// lanes run the same Stockham passes as above.  LDS: T sequence buffers (256 R 16 B).
+// lanes run the same Stockham passes as above.
+//
+// Sequence-buffer stride (elements): at least lds_seq_stride(M), and such that the T buffers
+// start in different LDS bank groups where the latitude-fastest accesses touch them.  With
+// 16-byte elements 8 lanes share an LDS cycle (4 of the 32 banks each), so the T latitudes
+// x 8/T rows of one cycle are conflict-free for a stride = 8/T mod 8; with 8-byte elements
+// 16 lanes share a cycle (2 banks each) and the stride is 16/T mod 16.
+constexpr int coop_stride(int M, int T, int lanes) {
+    const int want = T >= lanes ? 1 : lanes / T;
+    int s = lds_seq_stride(M);
+    while (s % lanes != want % lanes) ++s;
+    return s;
+}
+
 template <int M, int NTB>
 struct CoopGeom {
     using G = WaveGeom<M>;
     static constexpr int LQ = G::LQ, R = G::R;
-    static constexpr int NT = NTB;       // threads per block (128 or 256)
-    static constexpr int T = NT / LQ;   // latitudes per block
-    static constexpr size_t lds_bytes = (size_t)T * lds_seq_stride(M) * sizeof(double2);
+    static constexpr int NT = NTB;         // threads per block (128 or 256)
+    static constexpr int T = NT / LQ;      // latitudes per block
+    static constexpr int ROWS = NT / T;    // rows (points) per latitude-fastest access
+    static constexpr int STR = coop_stride(M, T, 8);
+    static constexpr size_t lds_bytes = (size_t)T * STR * sizeof(double2);
 };
 
+// X_k of latitudes lat0 .. lat0+T-1 of one field (Ff = the field's F) for a cooperative
+// block: R 16-byte loads per thread with the latitude fastest, all issued before any is used.
 template <int M, int NTB>
-__global__ __launch_bounds__(NTB) void fft_synth_coop(const double2 *__restrict__ F, long four_dist,
-                                                      double *__restrict__ V, long spat_dist,
-                                                      const double2 *__restrict__ roots, int nlat, int imlim, long sp,
-                                                      long stt, int lat_base, int lat_end) {
+__device__ __forceinline__ void coop_load_F(double2 (&xr)[WaveGeom<M>::R], const double2 *__restrict__ Ff,
+                                            int lat0, int lat_end, int nlat, int imlim) {
     using CG = CoopGeom<M, NTB>;
-    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = lds_seq_stride(M);
-    extern __shared__ double2 lds[];
-    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
-    // 1) X_k of the block's T latitudes into LDS, latitude fastest
-    {
-        // R loads per thread, all issued before the first LDS store
-        const int tl = threadIdx.x % T, row = threadIdx.x / T;
-        const bool live = lat0 + tl < lat_end;
-        const double2 *Ff = F + (size_t)blockIdx.y * four_dist + (live ? lat0 + tl : 0);
-        double2 *dst = lds + tl * STR;
-        double2 xr[R];
-        sfor<R>([&](auto ic) {
-            const int k = row + (NT / T) * decltype(ic)::value;
-            xr[decltype(ic)::value] = (live && k <= imlim) ? Ff[(size_t)k * nlat] : make_double2(0.0, 0.0);
-        });
-        sfor<R>([&](auto ic) { dst[lds_slot(row + (NT / T) * decltype(ic)::value)] = xr[decltype(ic)::value]; });
-    }
-    __syncthreads();
+    constexpr int R = CG::R, T = CG::T, ROWS = CG::ROWS;
+    const int tl = threadIdx.x % T, row = threadIdx.x / T;
+    const bool live = lat0 + tl < lat_end;
+    const unsigned off = 16u * (unsigned)(lat0 + tl + row * nlat), step = 16u * (unsigned)(ROWS * nlat);
+    sfor<R>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        xr[i] = (live && row + ROWS * i <= imlim) ? at(Ff, off + i * step) : make_double2(0.0, 0.0);
+    });
+}
+
+// Points of a cooperative block's latitude-fastest loads into the T LDS sequence buffers.
+template <int M, int NTB>
+__device__ __forceinline__ void coop_stage(const double2 (&xr)[WaveGeom<M>::R], double2 *lds) {
+    using CG = CoopGeom<M, NTB>;
+    const int tl = threadIdx.x % CG::T, row = threadIdx.x / CG::T;
+    double2 *dst = lds + tl * CG::STR;
+    sfor<CG::R>([&](auto ic) { dst[slot_at<CG::ROWS>(row, decltype(ic)::value)] = xr[decltype(ic)::value]; });
+}
+
+// Cooperative synthesis of one latitude group whose X_k are staged in LDS (after a barrier):
+// packed sequences, passes, z_n out to the field's V (Vf) with the latitude fastest.
+template <int M, int NTB>
+__device__ __forceinline__ void coop_synth_staged(double2 *lds, double *__restrict__ Vf,
+                                                  const double2 *__restrict__ roots, double2 wl, int imlim, long sp,
+                                                  long stt, int lat0, int lat_end) {
+    using CG = CoopGeom<M, NTB>;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, ROWS = CG::ROWS, STR = CG::STR;
     const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
     double2 *seq = lds + sq * STR;
-    const double2 wl = roots[l];
     double2 v[R];
     sfor<R>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const int k = l + LQ * j, km = M - k;
-        const double2 xk = (k <= imlim) ? seq[lds_slot(k)] : make_double2(0.0, 0.0);
-        const double2 xm = (k > 0 && km <= imlim) ? seq[lds_slot(km)] : make_double2(0.0, 0.0);
+        const double2 xk = (k <= imlim) ? seq[slot_at<LQ>(l, j)] : make_double2(0.0, 0.0);
+        const double2 xm = (k > 0 && km <= imlim) ? seq[slot_at<-LQ>(M - l, j)] : make_double2(0.0, 0.0);
         const double2 A = c_add(xk, c_conj(xm)), B = c_sub(xk, c_conj(xm));
         const double2 w = rot96<(96 / (2 * R)) * j, 1>(wl);
         const double2 wB = c_mul(w, B);
@@ -450,56 +508,73 @@ __global__ __launch_bounds__(NTB) void fft_synth_coop(const double2 *__restrict_
     });
     __syncthreads();
     wave_passes<M, 1, 0>(v, seq, l, roots);
-    // 2) z_n to LDS, then V out with the latitude fastest
-    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value]; });
+    sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value]; });
     __syncthreads();
-    {
-        const int tl = threadIdx.x % T, row = threadIdx.x / T;
-        if (lat0 + tl < lat_end) {
-            double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(lat0 + tl) * stt;
-            const double2 *src = lds + tl * STR;
-            sfor<R>([&](auto ic) {
-                const int n = row + (NT / T) * decltype(ic)::value;
-                const double2 z = src[lds_slot(n)];
-                Vf[(size_t)(2 * n) * sp] = z.x;
-                Vf[(size_t)(2 * n + 1) * sp] = z.y;
-            });
-        }
+    const int tl = threadIdx.x % T, row = threadIdx.x / T;
+    if (lat0 + tl < lat_end) {
+        const double2 *src = lds + tl * STR;
+        const unsigned off = 8u * (unsigned)((lat0 + tl) * stt + 2 * row * sp);
+        const unsigned step = 8u * (unsigned)(2 * ROWS * sp), odd = 8u * (unsigned)sp;
+        sfor<R>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            const double2 z = src[slot_at<ROWS>(row, i)];
+            at(Vf, off + i * step) = z.x;
+            at(Vf, off + i * step + odd) = z.y;
+        });
     }
 }
 
 template <int M, int NTB>
-__global__ __launch_bounds__(NTB) void fft_anal_coop(const double *__restrict__ V, long spat_dist,
-                                                     double2 *__restrict__ F, long four_dist,
-                                                     const double2 *__restrict__ roots, int nlat, int mmax, int imlim,
-                                                     long sp, long stt, int lat_base, int lat_end) {
+__global__ __launch_bounds__(NTB) void fft_synth_coop(const double2 *__restrict__ F, long four_dist,
+                                                      double *__restrict__ V, long spat_dist,
+                                                      const double2 *__restrict__ roots, int nlat, int imlim, long sp,
+                                                      long stt, int lat_base, int lat_end) {
     using CG = CoopGeom<M, NTB>;
-    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = lds_seq_stride(M);
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T;
     extern __shared__ double2 lds[];
     const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
-    // 1) z_n of the block's T latitudes into LDS, latitude fastest
-    {
-        const int tl = threadIdx.x % T, row = threadIdx.x / T;
-        const bool live = lat0 + tl < lat_end;
-        const double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(live ? lat0 + tl : 0) * stt;
-        double2 *dst = lds + tl * STR;
-        double2 zr[R];
-        sfor<R>([&](auto ic) {
-            const long n = row + (NT / T) * decltype(ic)::value;
-            zr[decltype(ic)::value] = live ? make_double2(Vf[(2 * n) * sp], Vf[(2 * n + 1) * sp]) : make_double2(0.0, 0.0);
-        });
-        sfor<R>([&](auto ic) { dst[lds_slot(row + (NT / T) * decltype(ic)::value)] = zr[decltype(ic)::value]; });
-    }
+    // X_k of the block's T latitudes into LDS, latitude fastest
+    double2 xr[R];
+    coop_load_F<M, NTB>(xr, F + (size_t)blockIdx.y * four_dist, lat0, lat_end, nlat, imlim);
+    coop_stage<M, NTB>(xr, lds);
     __syncthreads();
+    coop_synth_staged<M, NTB>(lds, V + (size_t)blockIdx.y * spat_dist, roots, roots[threadIdx.x % LQ], imlim, sp,
+                              stt, lat0, lat_end);
+}
+
+// z_n = V[2n] + i V[2n+1] of latitudes lat0 .. lat0+T-1 of one field (Vf = the field's V)
+// for a cooperative block: 2R 8-byte loads per thread with the latitude fastest.
+template <int M, int NTB>
+__device__ __forceinline__ void coop_load_V(double2 (&zr)[WaveGeom<M>::R], const double *__restrict__ Vf,
+                                            int lat0, int lat_end, long sp, long stt) {
+    using CG = CoopGeom<M, NTB>;
+    constexpr int R = CG::R, T = CG::T, ROWS = CG::ROWS;
+    const int tl = threadIdx.x % T, row = threadIdx.x / T;
+    const bool live = lat0 + tl < lat_end;
+    const unsigned off = 8u * (unsigned)((lat0 + tl) * stt + 2 * row * sp);
+    const unsigned step = 8u * (unsigned)(2 * ROWS * sp), odd = 8u * (unsigned)sp;
+    sfor<R>([&](auto ic) {
+        constexpr int i = decltype(ic)::value;
+        zr[i] = live ? make_double2(at(Vf, off + i * step), at(Vf, off + i * step + odd)) : make_double2(0.0, 0.0);
+    });
+}
+
+// Cooperative analysis of one latitude group whose z_n are staged in LDS (after a barrier):
+// passes, real-FFT post-twiddle, X_k out to the field's F (Ff) with the latitude fastest.
+template <int M, int NTB>
+__device__ __forceinline__ void coop_anal_staged(double2 *lds, double2 *__restrict__ Ff,
+                                                 const double2 *__restrict__ roots, double2 wl, int nlat, int mmax,
+                                                 int imlim, int lat0, int lat_end) {
+    using CG = CoopGeom<M, NTB>;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, ROWS = CG::ROWS, STR = CG::STR;
     const int sq = threadIdx.x / LQ, l = threadIdx.x % LQ;
     double2 *seq = lds + sq * STR;
     double2 v[R];
-    sfor<R>([&](auto jc) { v[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+    sfor<R>([&](auto jc) { v[decltype(jc)::value] = seq[slot_at<LQ>(l, decltype(jc)::value)]; });
     __syncthreads();
-    const double2 wl = roots[l];
     wave_passes<M, -1, 0>(v, seq, l, roots);
-    // 2) Z_n to LDS (Z_{M-k} sits in another lane), X_k in registers, X_k back to LDS
-    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value]; });
+    // Z_n to LDS (Z_{M-k} sits in another lane), X_k in registers, X_k back to LDS
+    sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value]; });
     __syncthreads();
     sfor<R>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
@@ -507,7 +582,7 @@ __global__ __launch_bounds__(NTB) void fft_anal_coop(const double *__restrict__ 
         double2 X = make_double2(0.0, 0.0);
         if (k <= imlim) {
             const double2 Zk = v[j];
-            const double2 Zm = seq[lds_slot(k == 0 ? 0 : M - k)];
+            const double2 Zm = seq[(j == 0 && l == 0) ? 0 : slot_at<-LQ>(M - l, j)];   // Z_{(M-k) mod M}
             const double2 A = make_double2(0.5 * (Zk.x + Zm.x), 0.5 * (Zk.y - Zm.y));
             const double2 D = make_double2(Zk.x - Zm.x, Zk.y + Zm.y);
             const double2 B = make_double2(0.5 * D.y, -0.5 * D.x);
@@ -519,34 +594,50 @@ __global__ __launch_bounds__(NTB) void fft_anal_coop(const double *__restrict__ 
     });
     __syncthreads();
     sfor<R>([&](auto jc) {
-        const int k = l + LQ * decltype(jc)::value;
-        if (k <= mmax) seq[lds_slot(k)] = v[decltype(jc)::value];
+        constexpr int j = decltype(jc)::value;
+        if (l + LQ * j <= mmax) seq[slot_at<LQ>(l, j)] = v[j];
     });
     __syncthreads();
-    // 3) F out, latitude fastest
-    {
-        const int tl = threadIdx.x % T, row = threadIdx.x / T;
-        if (lat0 + tl < lat_end) {
-            double2 *Ff = F + (size_t)blockIdx.y * four_dist + lat0 + tl;
-            const double2 *src = lds + tl * STR;
-            sfor<R>([&](auto ic) {
-                const int k = row + (NT / T) * decltype(ic)::value;
-                if (k <= mmax) Ff[(size_t)k * nlat] = src[lds_slot(k)];
-            });
-        }
+    const int tl = threadIdx.x % T, row = threadIdx.x / T;
+    if (lat0 + tl < lat_end) {
+        const double2 *src = lds + tl * STR;
+        const unsigned off = 16u * (unsigned)(lat0 + tl + row * nlat), step = 16u * (unsigned)(ROWS * nlat);
+        sfor<R>([&](auto ic) {
+            constexpr int i = decltype(ic)::value;
+            if (row + ROWS * i <= mmax) at(Ff, off + i * step) = src[slot_at<ROWS>(row, i)];
+        });
     }
 }
 
+template <int M, int NTB>
+__global__ __launch_bounds__(NTB) void fft_anal_coop(const double *__restrict__ V, long spat_dist,
+                                                     double2 *__restrict__ F, long four_dist,
+                                                     const double2 *__restrict__ roots, int nlat, int mmax, int imlim,
+                                                     long sp, long stt, int lat_base, int lat_end) {
+    using CG = CoopGeom<M, NTB>;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T;
+    extern __shared__ double2 lds[];
+    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
+    // z_n of the block's T latitudes into LDS, latitude fastest
+    double2 zr[R];
+    coop_load_V<M, NTB>(zr, V + (size_t)blockIdx.y * spat_dist, lat0, lat_end, sp, stt);
+    coop_stage<M, NTB>(zr, lds);
+    __syncthreads();
+    coop_anal_staged<M, NTB>(lds, F + (size_t)blockIdx.y * four_dist, roots, roots[threadIdx.x % LQ], nlat, mmax,
+                             imlim, lat0, lat_end);
+}
+
 // Cooperative kernels with 8-byte LDS staging (real and imaginary parts in turn): T
-// latitudes in half the LDS of fft_*_coop (256 threads, M = 1536: T = 4 in 52 KB, three
-// blocks per CU).  HBM accesses keep their full width (16-byte F, 8-byte V elements).
+// latitudes in half the LDS of fft_*_coop (256 threads, M = 1536: T = 4 in 52 KB; 512
+// threads: T = 8 in 105 KB).  HBM accesses keep their full width (16-byte F, 8-byte V).
 template <int M, int NTB>
 struct CoopSGeom {
     using G = WaveGeom<M>;
     static constexpr int LQ = G::LQ, R = G::R;
     static constexpr int NT = NTB;
     static constexpr int T = NT / LQ;
-    static constexpr int STR = lds_seq_stride(M);   // doubles per sequence buffer
+    static constexpr int ROWS = NT / T;
+    static constexpr int STR = coop_stride(M, T, 16);   // doubles per sequence buffer
     static constexpr size_t lds_bytes = (size_t)T * STR * sizeof(double);
 };
 
@@ -556,7 +647,7 @@ __global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict
                                                        const double2 *__restrict__ roots, int nlat, int imlim,
                                                        long sp, long stt, int lat_base, int lat_end) {
     using CG = CoopSGeom<M, NTB>;
-    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = CG::STR, ROWS = NT / T;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, STR = CG::STR, ROWS = CG::ROWS;
     extern __shared__ double ldsd[];
     const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
     const int tl = threadIdx.x % T, row = threadIdx.x / T;
@@ -567,26 +658,27 @@ __global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict
     // X_k of the block's latitudes, 16-byte loads with the latitude fastest
     double2 xr[R];
     {
-        const double2 *Ff = F + (size_t)blockIdx.y * four_dist + (live ? lat0 + tl : 0);
+        const double2 *Ff = F + (size_t)blockIdx.y * four_dist;
+        const unsigned off = 16u * (unsigned)(lat0 + tl + row * nlat), step = 16u * (unsigned)(ROWS * nlat);
         sfor<R>([&](auto ic) {
-            const int k = row + ROWS * decltype(ic)::value;
-            xr[decltype(ic)::value] = (live && k <= imlim) ? Ff[(size_t)k * nlat] : make_double2(0.0, 0.0);
+            constexpr int i = decltype(ic)::value;
+            xr[i] = (live && row + ROWS * i <= imlim) ? at(Ff, off + i * step) : make_double2(0.0, 0.0);
         });
     }
     // real parts: keep s = Re X_k + Re X_{M-k}, d = Re X_k - Re X_{M-k}
-    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = xr[decltype(ic)::value].x; });
+    sfor<R>([&](auto ic) { mine[slot_at<ROWS>(row, decltype(ic)::value)] = xr[decltype(ic)::value].x; });
     __syncthreads();
     double sr[R], dr[R];
     sfor<R>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const int k = l + LQ * j, km = M - k;
-        const double a = (k <= imlim) ? seq[lds_slot(k)] : 0.0;
-        const double b = (k > 0 && km <= imlim) ? seq[lds_slot(km)] : 0.0;
+        const double a = (k <= imlim) ? seq[slot_at<LQ>(l, j)] : 0.0;
+        const double b = (k > 0 && km <= imlim) ? seq[slot_at<-LQ>(M - l, j)] : 0.0;
         sr[j] = a + b;
         dr[j] = a - b;
     });
     __syncthreads();
-    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = xr[decltype(ic)::value].y; });
+    sfor<R>([&](auto ic) { mine[slot_at<ROWS>(row, decltype(ic)::value)] = xr[decltype(ic)::value].y; });
     __syncthreads();
     // Z_k = A + i w B, A = X_k + conj X_{M-k}, B = X_k - conj X_{M-k}, w = e^{i pi k/M}
     const double2 wl = roots[l];
@@ -594,8 +686,8 @@ __global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict
     sfor<R>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const int k = l + LQ * j, km = M - k;
-        const double a = (k <= imlim) ? seq[lds_slot(k)] : 0.0;
-        const double b = (k > 0 && km <= imlim) ? seq[lds_slot(km)] : 0.0;
+        const double a = (k <= imlim) ? seq[slot_at<LQ>(l, j)] : 0.0;
+        const double b = (k > 0 && km <= imlim) ? seq[slot_at<-LQ>(M - l, j)] : 0.0;
         const double2 A = make_double2(sr[j], a - b), B = make_double2(dr[j], a + b);
         const double2 w = rot96<(96 / (2 * R)) * j, 1>(wl);
         const double2 wB = c_mul(w, B);
@@ -604,31 +696,35 @@ __global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict
     __syncthreads();
     wave_passes_split<M, 1, 0>(v, seq, l, roots);
     // z_n out: real parts to rows 2n, then imaginary parts to rows 2n+1
-    double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(live ? lat0 + tl : 0) * stt;
-    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].x; });
+    const unsigned off = 8u * (unsigned)((live ? lat0 + tl : 0) * stt + 2 * row * sp);
+    const unsigned step = 8u * (unsigned)(2 * ROWS * sp), odd = 8u * (unsigned)sp;
+    double *Vf = V + (size_t)blockIdx.y * spat_dist;
+    sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value].x; });
     __syncthreads();
     if (live)
         sfor<R>([&](auto ic) {
-            const long n = row + ROWS * decltype(ic)::value;
-            Vf[(2 * n) * sp] = mine[lds_slot((int)n)];
+            constexpr int i = decltype(ic)::value;
+            at(Vf, off + i * step) = mine[slot_at<ROWS>(row, i)];
         });
     __syncthreads();
-    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].y; });
+    sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value].y; });
     __syncthreads();
     if (live)
         sfor<R>([&](auto ic) {
-            const long n = row + ROWS * decltype(ic)::value;
-            Vf[(2 * n + 1) * sp] = mine[lds_slot((int)n)];
+            constexpr int i = decltype(ic)::value;
+            at(Vf, off + i * step + odd) = mine[slot_at<ROWS>(row, i)];
         });
 }
 
+// With 256 threads (52 KB of LDS at M = 1536) three blocks fit in a CU's LDS; the register
+// budget of three waves per SIMD (168 VGPRs) lets them all be resident.
 template <int M, int NTB>
-__global__ __launch_bounds__(NTB) void fft_anal_coops(const double *__restrict__ V, long spat_dist,
+__global__ __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(NTB == 256 ? 3 : 1))) void fft_anal_coops(const double *__restrict__ V, long spat_dist,
                                                       double2 *__restrict__ F, long four_dist,
                                                       const double2 *__restrict__ roots, int nlat, int mmax,
                                                       int imlim, long sp, long stt, int lat_base, int lat_end) {
     using CG = CoopSGeom<M, NTB>;
-    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, NT = CG::NT, STR = CG::STR, ROWS = NT / T;
+    constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, STR = CG::STR, ROWS = CG::ROWS;
     extern __shared__ double ldsd[];
     const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
     const int tl = threadIdx.x % T, row = threadIdx.x / T;
@@ -639,24 +735,26 @@ __global__ __launch_bounds__(NTB) void fft_anal_coops(const double *__restrict__
     // z_n = V[2n] + i V[2n+1] of the block's latitudes, latitude fastest
     double zx[R], zy[R];
     {
-        const double *Vf = V + (size_t)blockIdx.y * spat_dist + (size_t)(live ? lat0 + tl : 0) * stt;
+        const double *Vf = V + (size_t)blockIdx.y * spat_dist;
+        const unsigned off = 8u * (unsigned)((live ? lat0 + tl : 0) * stt + 2 * row * sp);
+        const unsigned step = 8u * (unsigned)(2 * ROWS * sp), odd = 8u * (unsigned)sp;
         sfor<R>([&](auto ic) {
-            const long n = row + ROWS * decltype(ic)::value;
-            zx[decltype(ic)::value] = live ? Vf[(2 * n) * sp] : 0.0;
-            zy[decltype(ic)::value] = live ? Vf[(2 * n + 1) * sp] : 0.0;
+            constexpr int i = decltype(ic)::value;
+            zx[i] = live ? at(Vf, off + i * step) : 0.0;
+            zy[i] = live ? at(Vf, off + i * step + odd) : 0.0;
         });
     }
-    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = zx[decltype(ic)::value]; });
+    sfor<R>([&](auto ic) { mine[slot_at<ROWS>(row, decltype(ic)::value)] = zx[decltype(ic)::value]; });
     __syncthreads();
     double vx[R];
-    sfor<R>([&](auto jc) { vx[decltype(jc)::value] = seq[lds_slot(l + LQ * decltype(jc)::value)]; });
+    sfor<R>([&](auto jc) { vx[decltype(jc)::value] = seq[slot_at<LQ>(l, decltype(jc)::value)]; });
     __syncthreads();
-    sfor<R>([&](auto ic) { mine[lds_slot(row + ROWS * decltype(ic)::value)] = zy[decltype(ic)::value]; });
+    sfor<R>([&](auto ic) { mine[slot_at<ROWS>(row, decltype(ic)::value)] = zy[decltype(ic)::value]; });
     __syncthreads();
     double2 v[R];
     sfor<R>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
-        v[j] = make_double2(vx[j], seq[lds_slot(l + LQ * j)]);
+        v[j] = make_double2(vx[j], seq[slot_at<LQ>(l, j)]);
     });
     __syncthreads();
     const double2 wl = roots[l];
@@ -664,19 +762,19 @@ __global__ __launch_bounds__(NTB) void fft_anal_coops(const double *__restrict__
     // X_k = A + conj(w_k) B with A = (Z_k + conj Z_{M-k})/2, B = (Z_k - conj Z_{M-k})/(2i):
     // Z_{M-k} from LDS, real then imaginary parts
     double zmx[R];
-    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].x; });
+    sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value].x; });
     __syncthreads();
     sfor<R>([&](auto jc) {
-        const int k = l + LQ * decltype(jc)::value;
-        zmx[decltype(jc)::value] = seq[lds_slot(k == 0 ? 0 : M - k)];
+        constexpr int j = decltype(jc)::value;
+        zmx[j] = seq[(j == 0 && l == 0) ? 0 : slot_at<-LQ>(M - l, j)];
     });
     __syncthreads();
-    sfor<R>([&](auto jc) { seq[lds_slot(l + LQ * decltype(jc)::value)] = v[decltype(jc)::value].y; });
+    sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value].y; });
     __syncthreads();
     sfor<R>([&](auto jc) {
         constexpr int j = decltype(jc)::value;
         const int k = l + LQ * j;
-        const double2 Zm = make_double2(zmx[j], seq[lds_slot(k == 0 ? 0 : M - k)]);
+        const double2 Zm = make_double2(zmx[j], seq[(j == 0 && l == 0) ? 0 : slot_at<-LQ>(M - l, j)]);
         double2 X = make_double2(0.0, 0.0);
         if (k <= imlim) {
             const double2 Zk = v[j];
@@ -693,25 +791,26 @@ __global__ __launch_bounds__(NTB) void fft_anal_coops(const double *__restrict__
     // F out with 16-byte stores, latitude fastest: real parts through LDS, then imaginary parts
     double fx[R];
     sfor<R>([&](auto jc) {
-        const int k = l + LQ * decltype(jc)::value;
-        if (k <= mmax) seq[lds_slot(k)] = v[decltype(jc)::value].x;
+        constexpr int j = decltype(jc)::value;
+        if (l + LQ * j <= mmax) seq[slot_at<LQ>(l, j)] = v[j].x;
     });
     __syncthreads();
     sfor<R>([&](auto ic) {
-        const int k = row + ROWS * decltype(ic)::value;
-        fx[decltype(ic)::value] = (k <= mmax) ? mine[lds_slot(k)] : 0.0;
+        constexpr int i = decltype(ic)::value;
+        fx[i] = (row + ROWS * i <= mmax) ? mine[slot_at<ROWS>(row, i)] : 0.0;
     });
     __syncthreads();
     sfor<R>([&](auto jc) {
-        const int k = l + LQ * decltype(jc)::value;
-        if (k <= mmax) seq[lds_slot(k)] = v[decltype(jc)::value].y;
+        constexpr int j = decltype(jc)::value;
+        if (l + LQ * j <= mmax) seq[slot_at<LQ>(l, j)] = v[j].y;
     });
     __syncthreads();
     if (live) {
-        double2 *Ff = F + (size_t)blockIdx.y * four_dist + lat0 + tl;
+        double2 *Ff = F + (size_t)blockIdx.y * four_dist;
+        const unsigned off = 16u * (unsigned)(lat0 + tl + row * nlat), step = 16u * (unsigned)(ROWS * nlat);
         sfor<R>([&](auto ic) {
-            const int k = row + ROWS * decltype(ic)::value;
-            if (k <= mmax) Ff[(size_t)k * nlat] = make_double2(fx[decltype(ic)::value], mine[lds_slot(k)]);
+            constexpr int i = decltype(ic)::value;
+            if (row + ROWS * i <= mmax) at(Ff, off + i * step) = make_double2(fx[i], mine[slot_at<ROWS>(row, i)]);
         });
     }
 }
@@ -855,9 +954,19 @@ bool anal_coops(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int
     return false;
 }
 
+// The cooperative kernels address V and F with 32-bit byte offsets from each field's base:
+// false when a field's V (nlat rows of 2M points, strides sp, st) or F (rows k <= kmax) would
+// reach 4 GiB.
+static bool coop_offsets_fit(int nlat, int M, int kmax, long sp, long st) {
+    const double v_bytes = 8.0 * ((double)(nlat - 1) * (double)st + (double)(2 * M - 1) * (double)sp + 2.0);
+    const double f_bytes = 16.0 * ((double)kmax + 1.0) * (double)nlat;
+    return v_bytes < 4294967296.0 && f_bytes < 4294967296.0;
+}
+
 bool launch_fft_synth_coop(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
                            long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base,
                            int lat_end, int nthreads) {
+    if (!coop_offsets_fit(nlat, M, imlim, sp, st)) return false;
     if (nthreads == -256)
         return synth_coops<256>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
     if (nthreads == -512)
@@ -870,6 +979,7 @@ bool launch_fft_synth_coop(const DeviceTables *t, int nlat, int M, int imlim, in
 bool launch_fft_anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
                           long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
                           int lat_end, int nthreads) {
+    if (!coop_offsets_fit(nlat, M, std::max(mmax, imlim), sp, st)) return false;
     if (nthreads == -256)
         return anal_coops<256>(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base,
                                lat_end);

@@ -495,13 +495,13 @@ bool lds_fft_specialized(int M) {
 // / 128 threads, 5 / 6 the same with 8-byte LDS staging and 256 / 512 threads (sht_fft.hip).
 // The register kernel keeps each latitude's accesses separate, contiguous only in the
 // phi-contiguous layout (sp == 1).  In the theta-contiguous layout, measured at lmax=1023 with
-// 16 fields: synthesis 0.41 ms with kind 3 (LDS kernel 0.55), analysis 0.48 ms with kind 6
-// (8 latitudes per block, 64-byte V runs; LDS kernel 0.57).  kind (autotuner) or
-// SHTNS_AMD_FFT=wave|lds|coop|coop128|coops|coops512 override.
+// 16 fields, kind 5: synthesis 0.29 ms (two blocks per CU; kind 3 0.37, LDS kernel 0.55),
+// analysis 0.33 ms (three blocks per CU; kind 6 0.38, LDS kernel 0.57).
+// kind (autotuner) or SHTNS_AMD_FFT=wave|lds|coop|coop128|coops|coops512 override.
 // threads per block of the cooperative kinds (negative: 8-byte LDS staging)
 static int coop_threads(int kind) { return kind == 4 ? 128 : kind == 5 ? -256 : kind == 6 ? -512 : 256; }
 
-static int fft_choice(int M, long sp, int kind, bool synthesis) {
+static int fft_choice(int M, long sp, int kind) {
     static const char *env = getenv("SHTNS_AMD_FFT");
     int want = kind;
     if (!want && env)
@@ -516,7 +516,7 @@ static int fft_choice(int M, long sp, int kind, bool synthesis) {
     if (want >= 3 && want <= 6 && fft_coop_supported(M)) return want;
     if (want == 1 && fft_wave_supported(M)) return 1;
     if (sp == 1 && fft_wave_supported(M)) return 1;
-    if (fft_coop_supported(M)) return synthesis ? 3 : 6;
+    if (fft_coop_supported(M)) return 5;
     if (lds_fft_specialized(M)) return 2;
     return fft_wave_supported(M) ? 1 : 2;
 }
@@ -546,7 +546,7 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
     if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
-    const int choice = generic ? 2 : fft_choice(M, sp, kind, true);
+    const int choice = generic ? 2 : fft_choice(M, sp, kind);
     if (choice >= 3 && choice <= 6 &&
         launch_fft_synth_coop(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end,
                               coop_threads(choice)))
@@ -581,7 +581,7 @@ void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int im
     if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
-    const int choice = generic ? 2 : fft_choice(M, sp, kind, false);
+    const int choice = generic ? 2 : fft_choice(M, sp, kind);
     if (choice >= 3 && choice <= 6 &&
         launch_fft_anal_coop(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end,
                              coop_threads(choice)))
