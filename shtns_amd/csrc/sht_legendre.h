@@ -63,6 +63,15 @@ __device__ inline int xcd_cluster(int b, int nb, int gsize) {
     return (x + 8 * (s / gsize)) * gsize + s % gsize;
 }
 
+// v of quad lane SEL (DPP quad_perm: lane 4q + i reads lane 4q + sel_i, sel_i = (SEL >> 2i) & 3;
+// no LDS traffic).  All lanes of the wave must be active.
+template <int SEL>
+__device__ __forceinline__ double quad_perm(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), SEL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), SEL, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // ---------------------------------------------------------------------------------------
 // Contraction precision PREC (the recurrence, seeds, tables and the final sums stay FP64):
 //   0  FP64 matrix cores: v_mfma_f64_16x16x4f64 on z and the coefficients;
@@ -268,33 +277,49 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
         flush();
     }
 
-    // D[row][col = l%16] (row from Contract::d_row); row tile r covers latitudes tile*64 + 16r + row.
+    // Epilogue.  D[row][col]: the lane of column col = lane & 15 holds rows C::d_row(j, lane) of
+    // row tile r (latitudes tile*64 + 16r + row).  Columns are 4 fields x (E.re, E.im, O.re,
+    // O.im), so a quad of lanes is one field's (E, O) pairs: every lane reads its pair's O and
+    // E values (DPP), E lanes write north = E + x O and O lanes south = E - x O.
     const int col = lane & 15;
-    const int comp = col & 3;
-    const bool real_only = (im == 0);
-#pragma unroll
-    for (int c = 0; c < NCG; ++c) {
-        const int f = fbase + c * 4 + (col >> 2);
+    const int reim = col & 1;
+    const bool o_lane = (col & 2) != 0;
+    const bool zero_im = (im == 0) && reim;   // order 0 has no imaginary part
+    // every latitude of the tile and every field of the group exist (wave-uniform)
+    const bool full = (tile * 64 + 64 <= nlat_2) && (fbase + 4 * NCG <= nfields);
+    auto store_tile = [&](auto check) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
+            int row[4];
+            bool ok[4];
+            double xs[4], wz[4];
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const double mine = acc[r][c][j];
-                const double other = __shfl_xor(mine, 2);   // E <-> O partner
                 const int kr = tile * 64 + r * 16 + C::d_row(j, lane);
-                if (comp < 2 && kr < nlat_2 && f < nfields) {
-                    const double x = xk[kr];
-                    const double wmul = lat_weight ? wqk[kr] : 1.0;
-                    double n = (mine + x * other) * wmul;
-                    double sth = (mine - x * other) * wmul;
-                    if (real_only && comp == 1) { n = 0.0; sth = 0.0; }
-                    double *Ff = reinterpret_cast<double *>(F + (size_t)f * four_dist + (size_t)im * nlat);
-                    Ff[2 * kr + comp] = n;
-                    Ff[2 * (nlat - 1 - kr) + comp] = sth;
+                const int kc = min(kr, nlat_2 - 1);
+                const double x = xk[kc];
+                xs[j] = o_lane ? -x : x;
+                wz[j] = zero_im ? 0.0 : (lat_weight ? wqk[kc] : 1.0);
+                row[j] = o_lane ? (nlat - 1 - kr) : kr;
+                ok[j] = kr < nlat_2;
+            }
+#pragma unroll
+            for (int c = 0; c < NCG; ++c) {
+                const int f = fbase + c * 4 + (col >> 2);
+                double *Ff = reinterpret_cast<double *>(F + (size_t)min(f, nfields - 1) * four_dist +
+                                                        (size_t)im * nlat) + reim;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const double o = quad_perm<0xEE>(acc[r][c][j]);   // lanes (2, 3, 2, 3)
+                    const double e = quad_perm<0x44>(acc[r][c][j]);   // lanes (0, 1, 0, 1)
+                    const double val = fma(xs[j], o, e) * wz[j];
+                    if (!decltype(check)::value || (ok[j] && f < nfields)) Ff[2 * row[j]] = val;
                 }
             }
         }
-    }
+    };
+    if (full) store_tile(std::false_type{});
+    else store_tile(std::true_type{});
 }
 
 inline int synth_ncg(int nfields, int want = 0) {
