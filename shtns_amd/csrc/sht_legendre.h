@@ -377,14 +377,32 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int im = blockIdx.x;
-    const int fbase = blockIdx.y * 4 * NCG;
+    // block -> (order, field group), field groups fastest: blocks are dispatched in index
+    // order, so all field groups of the heavy low orders start first
+    const int ngr = gridDim.x / (mmax + 1);
+    const int im = blockIdx.x / ngr;
+    const int fbase = (blockIdx.x - im * ngr) * 4 * NCG;
 
     const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, vector_mode);
     const int nsteps = (im <= imlim) ? o.nsteps : 0;
     const int sofs = step_ofs[o.slot];
     const UniformTable2 abm(ab + sofs);   // uniform indices -> scalar loads
     const double4 *pmm = pm + sofs;
+
+    // (multi-tile waves keep their registers for the operands: scalar loads there)
+    if constexpr (NLR >= 3) cf_lds = 0;
+    // one-tile waves load the coefficient-table copy before the operands below: global loads
+    // complete in issue order, so its LDS store (before the first barrier) then waits only for
+    // these (multi-tile waves have no registers to spare and load it afterwards)
+    constexpr int NCF = (NLR == 1) ? 4 : 0;   // per thread; the rest of the table, if any, below
+    double2 cfr[NCF > 0 ? NCF : 1];
+    if (NCF > 0 && cf_lds) {
+#pragma unroll
+        for (int q = 0; q < NCF; ++q) {
+            const int e = threadIdx.x + q * 64 * NW;
+            cfr[q] = (e < nsteps) ? ab[sofs + e] : make_double2(0.0, 0.0);
+        }
+    }
 
     double za[NLR], zb[NLR], x2v[NLR], sdx[NLR], sdy[NLR];
     int i0v[NLR];
@@ -455,10 +473,14 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     my_imax = __builtin_amdgcn_readfirstlane(my_imax);
     if (lane == 0) s_ibeg[wave] = my_ibeg;
     if (threadIdx.x < 2 * NCOL) (&carry[0][0])[threadIdx.x] = 0.0;
-    // (multi-tile waves keep their registers for the operands: scalar loads there)
-    if constexpr (NLR >= 3) cf_lds = 0;
-    if (cf_lds)
-        for (int e = threadIdx.x; e < nsteps; e += 64 * NW) cfl[e] = ab[sofs + e];
+    if (cf_lds) {
+#pragma unroll
+        for (int q = 0; q < NCF; ++q) {
+            const int e = threadIdx.x + q * 64 * NW;
+            if (e < nsteps) cfl[e] = cfr[q];
+        }
+        for (int e = threadIdx.x + NCF * 64 * NW; e < nsteps; e += 64 * NW) cfl[e] = ab[sofs + e];   // rest, if any
+    }
     __syncthreads();
     int ibeg = nsteps;
     for (int w = 0; w < NW; ++w) ibeg = min(ibeg, s_ibeg[w]);
@@ -603,7 +625,7 @@ template <int NW, int NLR, int NCG, int PREC>
 void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                  int spec_ltop, hipStream_t s) {
     const DeviceTables *t = a.t;
-    dim3 grid(a.mmax + 1, (a.nfields + 4 * NCG - 1) / (4 * NCG));
+    dim3 grid((a.mmax + 1) * ((a.nfields + 4 * NCG - 1) / (4 * NCG)));
     // static LDS of the kernel, and the coefficient table if it fits beside it in 160 KB
     constexpr size_t NCOL = 16 * NCG;
     constexpr size_t static_lds = sizeof(double) * (NW * 16 * 65 + NW * 16 * (NCOL + 1) + 2 * NCOL) + sizeof(int) * NW;
