@@ -119,47 +119,43 @@ void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_
 //   qmix[g][slot steps][NCOL],  col = 16*cg + 4*field + (E.re, E.im, O.re, O.im),
 //   E_i = ue_i Q_{m+2i} + ue1_i Q_{m+2i+2},  O_i = uo_i Q_{m+2i+1}    (g = group of 4*NCG fields)
 // ---------------------------------------------------------------------------------------
+template <int NCOL>
 __global__ __launch_bounds__(256) void premix_kernel(const int *__restrict__ step_ofs, const double4 *__restrict__ pm,
                                                      const double2 *__restrict__ Q, long spec_dist, int lmax_spec,
-                                                     double *__restrict__ qmix, int nsteps_total, int ncol, int mmax,
-                                                     int mres, int ltr, int imlim, int nfields, int vector_mode,
+                                                     double *__restrict__ qmix, int nsteps_total, int mmax, int mres,
+                                                     int ltr, int imlim, int nfields, int vector_mode,
                                                      double mpos_scale) {
+    constexpr int FPG = NCOL / 4;   // fields per group; one thread per (step, field): 4 columns
     const int im = blockIdx.x;
     const int g = blockIdx.y;
     if (im > imlim) return;
     const OrderInfo o = order_info(im, lmax_spec, mmax, mres, ltr, vector_mode);
     const int nsteps = o.nsteps;
     const double4 *pmm = pm + step_ofs[o.slot];
-    double *dst = qmix + ((size_t)g * nsteps_total + step_ofs[o.slot]) * ncol;
+    double *dst = qmix + ((size_t)g * nsteps_total + step_ofs[o.slot]) * NCOL;
     const double qscale = (im == 0) ? 1.0 : mpos_scale;
-    const int fpg = ncol / 4;   // fields per group
     // rows nsteps .. end of the slot are written as zeros: the Legendre kernel reads them
     // unconditionally (clamped to the slot) when it runs past the last step
     const int slot_rows = step_ofs[o.slot + 1] - step_ofs[o.slot];
-    for (int e = blockIdx.z * blockDim.x + threadIdx.x; e < slot_rows * ncol; e += gridDim.z * blockDim.x) {
-        const int ii = e / ncol, col = e - ii * ncol;
-        const int f = g * fpg + (col >> 2);
-        const int comp = col & 3;
-        double v = 0.0;
-        if (ii < nsteps && f < nfields && !(o.real_only && (comp & 1))) {
+    for (int e = blockIdx.z * blockDim.x + threadIdx.x; e < slot_rows * FPG; e += gridDim.z * blockDim.x) {
+        const int ii = e / FPG, fi = e % FPG;
+        const int f = g * FPG + fi;
+        double4 out = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (ii < nsteps && f < nfields) {
             const double2 *Qf = Q + (size_t)f * spec_dist + o.spec_base;
             const int le = o.mfun + 2 * ii;
             const double4 cf = pmm[ii];
-            if (comp < 2) {
-                const double2 q0 = Qf[le];
-                double a = (comp == 0) ? q0.x : q0.y, b = 0.0;
-                if (le + 2 <= ltr) {
-                    const double2 q2 = Qf[le + 2];
-                    b = (comp == 0) ? q2.x : q2.y;
-                }
-                v = cf.x * a + cf.y * b;
-            } else if (le + 1 <= ltr) {
-                const double2 q1 = Qf[le + 1];
-                v = cf.z * ((comp == 2) ? q1.x : q1.y);
-            }
-            v *= qscale;
+            const double2 q0 = Qf[le];
+            const double2 q1 = (le + 1 <= ltr) ? Qf[le + 1] : make_double2(0.0, 0.0);
+            const double2 q2 = (le + 2 <= ltr) ? Qf[le + 2] : make_double2(0.0, 0.0);
+            // columns (E.re, E.im, O.re, O.im): E = ue Q_l + ue1 Q_{l+2}, O = uo Q_{l+1}
+            out.x = (cf.x * q0.x + cf.y * q2.x) * qscale;
+            out.y = (cf.x * q0.y + cf.y * q2.y) * qscale;
+            out.z = (cf.z * q1.x) * qscale;
+            out.w = (cf.z * q1.y) * qscale;
+            if (o.real_only) out.y = out.w = 0.0;
         }
-        dst[e] = v;
+        *reinterpret_cast<double4 *>(dst + (size_t)ii * NCOL + 4 * fi) = out;
     }
 }
 
@@ -180,9 +176,14 @@ void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, in
     const DeviceTables *t = a.t;
     const int ncg = synth_ncg(a.nfields, a.synth_ncg);
     const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
-    dim3 grid(a.mmax + 1, ng, 4);
-    hipLaunchKernelGGL(premix_kernel, grid, dim3(256), 0, s, t->step_ofs, t->pm, Qlm, spec_dist, spec_ltop, qmix,
-                       nsteps_total, 16 * ncg, a.mmax, a.mres, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale);
+    dim3 grid(a.mmax + 1, ng, 2);
+#define PREMIX(NC)                                                                                             \
+    hipLaunchKernelGGL(premix_kernel<NC>, grid, dim3(256), 0, s, t->step_ofs, t->pm, Qlm, spec_dist, spec_ltop, qmix, \
+                       nsteps_total, a.mmax, a.mres, a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale)
+    if (ncg == 4) PREMIX(64);
+    else if (ncg == 2) PREMIX(32);
+    else PREMIX(16);
+#undef PREMIX
 }
 
 void launch_legendre_synth_tiles(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
