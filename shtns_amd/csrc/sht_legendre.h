@@ -499,6 +499,41 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     // post-mix work split: thread handles elements e = threadIdx.x + j*64*NW (j < NE) of the
     // 16 x NCOL block; their coefficient rows are prefetched before the MFMA phase.
     constexpr int NE = (16 * NCOL + 64 * NW - 1) / (64 * NW);
+    auto cf_at = [&](auto from_lds, int idx) {
+        idx = min(idx, nsteps - 1);
+        if constexpr (decltype(from_lds)::value) return cfl[idx];   // uniform: LDS broadcast
+        else return abm[idx];                                        // uniform: scalar load
+    };
+    // recurrence of the 16-step group from ig of tile r into this wave's tile buffer
+    auto recur_plain = [&](int r, int ig, const double2 *cfg) {
+        if (ig <= my_imax) {   // some lane of this wave starts its recurrence here
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                if (ig + s == i0v[r]) { za[r] = sdx[r]; zb[r] = sdy[r]; }
+                C::store_z(&ytile[wave][s][lane], zb[r]);
+                const double zn = fma(fma(cfg[s].x, x2v[r], cfg[s].y), zb[r], za[r]);
+                za[r] = zb[r];
+                zb[r] = zn;
+            }
+        } else {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                C::store_z(&ytile[wave][s][lane], zb[r]);
+                const double zn = fma(fma(cfg[s].x, x2v[r], cfg[s].y), zb[r], za[r]);
+                za[r] = zb[r];
+                zb[r] = zn;
+            }
+        }
+    };
+    auto load_cf16 = [&](double2 *cf, int ig) {
+        if (cf_lds) {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) cf[s] = cf_at(std::true_type{}, ig + s);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 16; ++s) cf[s] = cf_at(std::false_type{}, ig + s);
+        }
+    };
     int b = 0;
     for (int i = ibeg; i < nsteps; i += 16, ++b) {
         double pcx[NE], pcz[NE], pcp[NE];
@@ -512,52 +547,30 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             pcz[j] = cf4.z;
             pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
         }
-        double2 cf[16];
-        if (cf_lds) {
-#pragma unroll
-            for (int s = 0; s < 16; ++s) cf[s] = cfl[min(i + s, nsteps - 1)];   // uniform: LDS broadcast
-        } else {
-#pragma unroll
-            for (int s = 0; s < 16; ++s) cf[s] = abm[min(i + s, nsteps - 1)];   // uniform: scalar loads
-        }
         typename C::acc_t acc[NCG], acc2[NCG];   // two chains per column group (even / odd K-groups)
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] = acc2[c] = C::zero();
+        {
+            double2 cf[16];
+            load_cf16(cf, i);
 #pragma unroll
-        for (int r = 0; r < NLR; ++r) {
-            if (i <= my_imax) {   // some lane of this wave starts its recurrence here
+            for (int r = 0; r < NLR; ++r) {
+                recur_plain(r, i, cf);
+                typename C::a_t a[16];
+                __builtin_amdgcn_s_waitcnt(0xc07f);
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int s = 0; s < 16; ++s) {
-                    const int ii = i + s;
-                    if (ii == i0v[r]) { za[r] = sdx[r]; zb[r] = sdy[r]; }
-                    C::store_z(&ytile[wave][s][lane], zb[r]);
-                    const double zn = fma(fma(cf[s].x, x2v[r], cf[s].y), zb[r], za[r]);
-                    za[r] = zb[r];
-                    zb[r] = zn;
-                }
-            } else {
+                for (int kg = 0; kg < 16; ++kg) a[kg] = C::load_z(&ytile[wave][lane & 15][4 * kg + (lane >> 4)]);
+                __builtin_amdgcn_s_waitcnt(0xc07f);   // all A operands in registers before the tile is rewritten
+                __builtin_amdgcn_wave_barrier();
 #pragma unroll
-                for (int s = 0; s < 16; ++s) {
-                    C::store_z(&ytile[wave][s][lane], zb[r]);
-                    const double zn = fma(fma(cf[s].x, x2v[r], cf[s].y), zb[r], za[r]);
-                    za[r] = zb[r];
-                    zb[r] = zn;
-                }
+                for (int kg = 0; kg < 16; kg += 2)
+#pragma unroll
+                    for (int c = 0; c < NCG; ++c) {
+                        acc[c] = C::mma(a[kg], Ereg[r][c][kg], acc[c]);
+                        acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
+                    }
             }
-            __builtin_amdgcn_s_waitcnt(0xc07f);
-            __builtin_amdgcn_wave_barrier();
-            typename C::a_t a[16];
-#pragma unroll
-            for (int kg = 0; kg < 16; ++kg) a[kg] = C::load_z(&ytile[wave][lane & 15][4 * kg + (lane >> 4)]);
-            __builtin_amdgcn_s_waitcnt(0xc07f);   // all A operands in registers before the tile is rewritten
-            __builtin_amdgcn_wave_barrier();
-#pragma unroll
-            for (int kg = 0; kg < 16; kg += 2)
-#pragma unroll
-                for (int c = 0; c < NCG; ++c) {
-                    acc[c] = C::mma(a[kg], Ereg[r][c][kg], acc[c]);
-                    acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
-                }
         }
 #pragma unroll
         for (int c = 0; c < NCG; ++c) acc[c] += acc2[c];
