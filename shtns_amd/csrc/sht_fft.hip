@@ -757,10 +757,10 @@ __global__ __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(NTB == 256 
         v[j] = make_double2(vx[j], seq[slot_at<LQ>(l, j)]);
     });
     __syncthreads();
-    const double2 wl = roots[l];
     wave_passes_split<M, -1, 0>(v, seq, l, roots);
     // X_k = A + conj(w_k) B with A = (Z_k + conj Z_{M-k})/2, B = (Z_k - conj Z_{M-k})/(2i):
     // Z_{M-k} from LDS, real then imaginary parts
+    const double2 wl = roots[l];   // loaded here, not before the passes: their register peak
     double zmx[R];
     sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value].x; });
     __syncthreads();
