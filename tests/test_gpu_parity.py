@@ -110,6 +110,11 @@ def test_batch_equals_single(device):
     qa = s.analys(torch.from_numpy(vb).to(device)).cpu().numpy()
     for f in range(9):
         assert rel_l2(qa[f], qb[f]) <= 1e-13
+        # 9 fields run as two field groups (8 + 1) of one launch; each field's result is the
+        # single-field one bit for bit, and the oracle's
+        q1 = s.analys(torch.from_numpy(vb[f]).to(device)).cpu().numpy()
+        assert np.array_equal(qa[f], q1), f
+        assert rel_l2(qa[f], o.analys(vb[f])) <= TOL, f
 
 
 def test_phi_contiguous_and_host_api(device):
