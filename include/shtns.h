@@ -117,6 +117,11 @@ void SHsphtor_to_spat(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp);
 void SHsph_to_spat(shtns_cfg, cplx *Slm, double *Vt, double *Vp);
 void SHtor_to_spat(shtns_cfg, cplx *Tlm, double *Vt, double *Vp);
 
+/* 3-component transforms: radial scalar Q and tangential spheroidal/toroidal S, T
+   (shtns.h:303, 306) */
+void spat_to_SHqst(shtns_cfg, double *Vr, double *Vt, double *Vp, cplx *Qlm, cplx *Slm, cplx *Tlm);
+void SHqst_to_spat(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double *Vr, double *Vt, double *Vp);
+
 /* truncated transforms at degree ltr <= lmax (shtns.h:315-324) */
 void spat_to_SH_l(shtns_cfg, double *Vr, cplx *Qlm, int ltr);
 void SH_to_spat_l(shtns_cfg, cplx *Qlm, double *Vr, int ltr);
@@ -124,6 +129,8 @@ void SHsphtor_to_spat_l(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp,
 void SHsph_to_spat_l(shtns_cfg, cplx *Slm, double *Vt, double *Vp, int ltr);
 void SHtor_to_spat_l(shtns_cfg, cplx *Tlm, double *Vt, double *Vp, int ltr);
 void spat_to_SHsphtor_l(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr);
+void spat_to_SHqst_l(shtns_cfg, double *Vr, double *Vt, double *Vp, cplx *Qlm, cplx *Slm, cplx *Tlm, int ltr);
+void SHqst_to_spat_l(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double *Vr, double *Vt, double *Vp, int ltr);
 
 /* point evaluation (shtns.h:350) */
 double SH_to_point(shtns_cfg, cplx *Qlm, double cost, double phi);

@@ -27,6 +27,10 @@ void cu_SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr, int ltr);            
 void cu_SH_to_spat_float(shtns_cfg shtns, cplx_f *Qlm, float *Vr, int ltr);                /* :44 */
 void cu_spat_to_SHsphtor(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr); /* :46 */
 void cu_SHsphtor_to_spat(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp, int ltr); /* :48 */
+void cu_spat_to_SHqst(shtns_cfg, double *Vr, double *Vt, double *Vp, cplx *Qlm, cplx *Slm, cplx *Tlm,
+                      int ltr);                                                             /* :50 */
+void cu_SHqst_to_spat(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double *Vr, double *Vt, double *Vp,
+                      int ltr);                                                             /* :52 */
 
 /* ---- initialization (shtns_cuda.h:63-72, sht_gpu.cu:227-336) ---- */
 shtns_cfg cushtns_clone(shtns_cfg shtns, hipStream_t compute_stream, hipStream_t transfer_stream); /* :63 */

@@ -207,14 +207,32 @@ class sht:
         _lib.load().spat_to_SHsphtor_l(self._cfg, _ptr(Vt), _ptr(Vp), _ptr(Slm), _ptr(Tlm), self.lmax if ltr < 0 else ltr)
         _lib.check_error("spat_to_SHsphtor")
 
+    def SHqst_to_spat(self, Qlm, Slm, Tlm, Vr, Vt, Vp, ltr=-1):
+        """3-component synthesis: radial Qlm -> Vr, tangential Slm, Tlm -> Vt, Vp
+        (shtns_numpy.i:376-380)."""
+        self._need_grid()
+        _lib.load().SHqst_to_spat_l(self._cfg, _ptr(Qlm), _ptr(Slm), _ptr(Tlm), _ptr(Vr), _ptr(Vt), _ptr(Vp),
+                                    self.lmax if ltr < 0 else ltr)
+        _lib.check_error("SHqst_to_spat")
+
+    def spat_to_SHqst(self, Vr, Vt, Vp, Qlm, Slm, Tlm, ltr=-1):
+        """3-component analysis (shtns_numpy.i:371-375)."""
+        self._need_grid()
+        _lib.load().spat_to_SHqst_l(self._cfg, _ptr(Vr), _ptr(Vt), _ptr(Vp), _ptr(Qlm), _ptr(Slm), _ptr(Tlm),
+                                    self.lmax if ltr < 0 else ltr)
+        _lib.check_error("spat_to_SHqst")
+
     # ---- generic entry points (numpy or torch) ----
     def synth(self, *arg, ltr=-1):
-        """v = synth(qlm) ; vt, vp = synth(slm, tlm)   (shtns_numpy.i:396-420)."""
+        """v = synth(qlm) ; vt, vp = synth(slm, tlm) ; vr, vt, vp = synth(qlm, slm, tlm)
+        (shtns_numpy.i:396-426)."""
         self._need_grid()
         n = len(arg)
-        if n not in (1, 2):
-            raise RuntimeError("1 or 2 spectral arrays required.")
+        if n not in (1, 2, 3):
+            raise RuntimeError("1, 2 or 3 spectral arrays required.")
         if _is_torch(arg[0]):
+            if n == 3:   # radial scalar, then the tangential vector pair (one stream)
+                return (self._synth_torch(arg[:1], ltr),) + tuple(self._synth_torch(arg[1:], ltr))
             return self._synth_torch(arg, ltr)
         q = []
         for a in arg:
@@ -227,16 +245,23 @@ class sht:
             self.SH_to_spat(q[0], vr, ltr)
             return vr
         vt, vp = np.empty(self.spat_shape), np.empty(self.spat_shape)
+        if n == 3:
+            vr = np.empty(self.spat_shape)
+            self.SHqst_to_spat(q[0], q[1], q[2], vr, vt, vp, ltr)
+            return vr, vt, vp
         self.SHsphtor_to_spat(q[0], q[1], vt, vp, ltr)
         return vt, vp
 
     def analys(self, *arg, ltr=-1):
-        """qlm = analys(v) ; slm, tlm = analys(vt, vp)."""
+        """qlm = analys(v) ; slm, tlm = analys(vt, vp) ; qlm, slm, tlm = analys(vr, vt, vp)
+        (shtns_numpy.i:427-457)."""
         self._need_grid()
         n = len(arg)
-        if n not in (1, 2):
-            raise RuntimeError("1 or 2 spatial arrays required.")
+        if n not in (1, 2, 3):
+            raise RuntimeError("1, 2 or 3 spatial arrays required.")
         if _is_torch(arg[0]):
+            if n == 3:
+                return (self._analys_torch(arg[:1], ltr),) + tuple(self._analys_torch(arg[1:], ltr))
             return self._analys_torch(arg, ltr)
         v = []
         for a in arg:
@@ -247,6 +272,10 @@ class sht:
             self.spat_to_SH(v[0], q, ltr)
             return q
         s, t = self.spec_array(), self.spec_array()
+        if n == 3:
+            q = self.spec_array()
+            self.spat_to_SHqst(v[0], v[1], v[2], q, s, t, ltr)
+            return q, s, t
         self.spat_to_SHsphtor(v[0], v[1], s, t, ltr)
         return s, t
 

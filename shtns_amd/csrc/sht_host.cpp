@@ -1186,4 +1186,39 @@ extern "C" void SHtor_to_spat_l(shtns_cfg c, cplx *T, double *Vt, double *Vp, in
     SHsphtor_to_spat_l(c, nullptr, T, Vt, Vp, ltr);
 }
 extern "C" void SHsph_to_spat(shtns_cfg c, cplx *S, double *Vt, double *Vp) { SHsph_to_spat_l(c, S, Vt, Vp, c->lmax); }
+
+/* 3-component transforms (radial scalar + tangential vector): the scalar transform of the
+   radial component, then the vector transform of the tangential ones, on the same stream
+   (SHTns composes them the same way on the GPU, sht_gpu.cu:531-577; host-pointer forms
+   sht_com.c:42-48, 77-83). */
+extern "C" void cu_SHqst_to_spat(shtns_cfg c, cplx *Q, cplx *S, cplx *T, double *Vr, double *Vt, double *Vp,
+                                 int ltr) {
+    cu_SH_to_spat(c, Q, Vr, ltr);
+    if (g_last_err) return;
+    cu_SHsphtor_to_spat(c, S, T, Vt, Vp, ltr);
+}
+extern "C" void cu_spat_to_SHqst(shtns_cfg c, double *Vr, double *Vt, double *Vp, cplx *Q, cplx *S, cplx *T,
+                                 int ltr) {
+    cu_spat_to_SH(c, Vr, Q, ltr);
+    if (g_last_err) return;
+    cu_spat_to_SHsphtor(c, Vt, Vp, S, T, ltr);
+}
+extern "C" void SHqst_to_spat_l(shtns_cfg c, cplx *Q, cplx *S, cplx *T, double *Vr, double *Vt, double *Vp,
+                                int ltr) {
+    SH_to_spat_l(c, Q, Vr, ltr);
+    if (g_last_err) return;
+    SHsphtor_to_spat_l(c, S, T, Vt, Vp, ltr);
+}
+extern "C" void spat_to_SHqst_l(shtns_cfg c, double *Vr, double *Vt, double *Vp, cplx *Q, cplx *S, cplx *T,
+                                int ltr) {
+    spat_to_SH_l(c, Vr, Q, ltr);
+    if (g_last_err) return;
+    spat_to_SHsphtor_l(c, Vt, Vp, S, T, ltr);
+}
+extern "C" void SHqst_to_spat(shtns_cfg c, cplx *Q, cplx *S, cplx *T, double *Vr, double *Vt, double *Vp) {
+    SHqst_to_spat_l(c, Q, S, T, Vr, Vt, Vp, c->lmax);
+}
+extern "C" void spat_to_SHqst(shtns_cfg c, double *Vr, double *Vt, double *Vp, cplx *Q, cplx *S, cplx *T) {
+    spat_to_SHqst_l(c, Vr, Vt, Vp, Q, S, T, c->lmax);
+}
 extern "C" void SHtor_to_spat(shtns_cfg c, cplx *T, double *Vt, double *Vp) { SHtor_to_spat_l(c, T, Vt, Vp, c->lmax); }
