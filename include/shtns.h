@@ -5,9 +5,10 @@
  * Each declaration names the SHTns interface it stands in for (file:line of shtns.h
  * in SHTns 3.5.2).
  *
- * Scope: Gauss grids, the scalar transforms SH_to_spat/spat_to_SH and the 2D vector
- * transforms SHsphtor_to_spat/spat_to_SHsphtor (plus their _l truncated forms), computed
- * on the GPU by hand-written HIP kernels.  Host-pointer transforms copy to the device,
+ * Scope: Gauss and regular (Fejer) grids; the scalar transforms SH_to_spat/spat_to_SH, the
+ * 2D vector transforms SHsphtor_to_spat/spat_to_SHsphtor and the 3-component transforms
+ * SHqst_to_spat/spat_to_SHqst (plus their _l truncated forms), computed on the GPU by
+ * hand-written HIP kernels.  Host-pointer transforms copy to the device,
  * run there and copy back; the device-pointer variants are in shtns_hip.h.
  */
 #ifndef SHTNS_AMD_SHTNS_H
@@ -36,8 +37,9 @@ enum shtns_norm { sht_orthonormal, sht_fourpi, sht_schmidt, sht_for_rotations };
 #define SHT_NO_CS_PHASE (256 * 4)
 #define SHT_REAL_NORM (256 * 8)
 
-/* grid / algorithm selectors (shtns.h:58-75).  Gauss grids are implemented:
-   sht_gauss, sht_auto, sht_quick_init and sht_gauss_fly all select the Gauss grid. */
+/* grid / algorithm selectors (shtns.h:58-75).  sht_gauss, sht_auto, sht_quick_init and
+   sht_gauss_fly select the Gauss grid; sht_reg_fast and sht_reg_dct the regular grid without
+   poles (equispaced theta, Fejer quadrature).  sht_reg_poles is not implemented. */
 enum shtns_type { sht_gauss, sht_auto, sht_reg_fast, sht_reg_dct, sht_quick_init, sht_reg_poles, sht_gauss_fly };
 #define SHT_NATIVE_LAYOUT 0
 #define SHT_THETA_CONTIGUOUS 256

@@ -63,6 +63,13 @@ def load():
         raise OSError(
             f"shtns_amd: {LIB_PATH} is missing; build it with `make -C shtns_amd/csrc` "
             "(or __graft_entry__.build()).  There is no fallback implementation.")
+    # torch ships its own HIP runtime.  With both in one process, torch's must be loaded
+    # first: if this library initializes the GPU before torch is imported, torch no longer
+    # sees the GPU afterwards (measured).  Importing torch (not initializing it) suffices.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
     CFG = ctypes.POINTER(ShtnsInfo)

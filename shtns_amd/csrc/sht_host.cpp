@@ -108,6 +108,31 @@ extern "C" void shtns_print_cfg(shtns_cfg c) {
 // ------------------------------------------------------------------------------------------
 // host math
 // ------------------------------------------------------------------------------------------
+// Regular grid without the poles (SHTns's sht_reg_fast / sht_reg_dct grid, sht_init.c:580-588):
+// theta_k = (k + 1/2) pi / n, north first, with Fejer's first quadrature rule
+//   w_k = (2/n) [1 - 2 sum_{j=1}^{n/2} cos(2 j theta_k) / (4 j^2 - 1)],
+// exact for polynomials in cos(theta) of degree < n (Fejer 1933; Waldvogel, BIT 46 (2006)).
+// The cosines cos(2 j theta_k) come from the Chebyshev recurrence in long double.
+static void fejer1_grid(double *x, double *st, double *w, int n) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < (n + 1) / 2; ++k) {
+        const long double th = pi * (k + 0.5L) / n;
+        const long double c = cosl(th), s = sinl(th);
+        x[k] = (double)c;
+        x[n - 1 - k] = (double)-c;
+        st[k] = st[n - 1 - k] = (double)s;
+        const long double c2 = cosl(2 * th);
+        long double tm = 1.0L, t = c2, sum = 0.0L;   // cos(2 (j-1) theta), cos(2 j theta)
+        for (int j = 1; j <= n / 2; ++j) {
+            sum += t / (4.0L * j * j - 1.0L);
+            const long double tn = 2 * c2 * t - tm;
+            tm = t;
+            t = tn;
+        }
+        w[k] = w[n - 1 - k] = (double)(2.0L / n * (1.0L - 2.0L * sum));
+    }
+}
+
 // Gauss-Legendre nodes x (descending, north first) and weights (sum 2), Newton iteration
 // from the same initial guess as SHTns (sht_legendre.c:763-811).
 static void gauss_nodes(double *x, double *st, double *w, int n) {
@@ -359,13 +384,15 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     clear_error();
     const int grid = (int)flags & 255;
     const int layout = (int)flags & ~255;
-    if (grid != sht_gauss && grid != sht_auto && grid != sht_quick_init && grid != sht_gauss_fly) {
-        set_error(1, "only Gauss grids are implemented (grid type %d)", grid);
+    const bool regular = (grid == sht_reg_fast || grid == sht_reg_dct);   // Fejer grid, no poles
+    if (!regular && grid != sht_gauss && grid != sht_auto && grid != sht_quick_init && grid != sht_gauss_fly) {
+        set_error(1, "grid type %d is not implemented (Gauss and regular grids without poles are)", grid);
         return 0;
     }
     if (nl_order <= 0) nl_order = 1;
-    if (*nlat <= 0) {   // choose sizes like SHTns: nlat > (nl_order+1)*lmax/2, nphi > (nl_order+1)*mmax
-        int n = ((nl_order + 1) * c->lmax) / 2 + 2;
+    if (*nlat <= 0) {   // choose sizes like SHTns: Gauss nlat > (nl_order+1)*lmax/2, regular
+                        // nlat >= (nl_order+1)*(lmax+1); nphi > (nl_order+1)*mmax
+        int n = regular ? (nl_order + 1) * (c->lmax + 1) : ((nl_order + 1) * c->lmax) / 2 + 2;
         n += (n & 1);
         *nlat = n;
     }
@@ -407,7 +434,8 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     c->ct = new double[2 * (size_t)c->nlat];
     c->st = c->ct + c->nlat;
     std::vector<double> w(c->nlat);
-    gauss_nodes(c->ct, c->st, w.data(), c->nlat);
+    if (regular) fejer1_grid(c->ct, c->st, w.data(), c->nlat);
+    else gauss_nodes(c->ct, c->st, w.data(), c->nlat);
     {   // quadrature sanity sums (sht_init.c:607-624)
         double s = 0, x2 = 0, st2 = 0;
         for (int i = 0; i < c->nlat_2; ++i) {
