@@ -33,3 +33,11 @@ def test_reg_poles_rejected():
     s = shtns_amd.sht(31)
     with pytest.raises(RuntimeError):
         s.set_grid(64, 128, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_reg_poles, polar_opt=0.0)
+
+
+def test_allow_padding_adds_none():
+    """SHT_ALLOW_PADDING permits padding; none is added (nspat = nlat * nphi)."""
+    s = shtns_amd.sht(31)
+    s.set_grid(64, 128, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.SHT_ALLOW_PADDING | shtns_amd.sht_gauss,
+               polar_opt=0.0)
+    assert s.nlat == 64 and s.nphi == 128 and s.nspat == 64 * 128
