@@ -7,7 +7,8 @@
  *
  * Scope: Gauss and regular (Fejer) grids; the scalar transforms SH_to_spat/spat_to_SH, the
  * 2D vector transforms SHsphtor_to_spat/spat_to_SHsphtor and the 3-component transforms
- * SHqst_to_spat/spat_to_SHqst (plus their _l truncated forms), computed on the GPU by
+ * SHqst_to_spat/spat_to_SHqst (plus their _l truncated forms) and the complex-valued
+ * scalar transforms SH_to_spat_cplx/spat_cplx_to_SH, computed on the GPU by
  * hand-written HIP kernels.  Host-pointer transforms copy to the device,
  * run there and copy back; the device-pointer variants are in shtns_hip.h.
  */
@@ -73,6 +74,9 @@ struct shtns_info {
 /* index macros (shtns.h:104-106, 120) */
 #define LiM(shtns, l, im) ((((im) * (2 * shtns->lmax + 2 - ((im) + 1) * shtns->mres)) >> 1) + (l))
 #define LM(shtns, l, m) ((((((unsigned short)(m)) / shtns->mres) * (2 * shtns->lmax + 2 - ((m) + shtns->mres))) >> 1) + (l))
+/* index of (l, m), -l <= m <= l, in a complex-valued field's spectrum, mres = 1 (shtns.h:114) */
+#define LM_cplx(shtns, l, m) \
+    (((l) <= shtns->mmax) ? (l) * ((l) + 1) + (m) : shtns->mmax * (2 * (l) - shtns->mmax) + (l) + (m))
 #define NSPAT_ALLOC(shtns) (shtns->nspat)
 #define PHI_DEG(shtns, ip) (360. / (shtns->nphi * shtns->mres)) * (ip)
 #define PHI_RAD(shtns, ip) (2. * M_PI / (shtns->nphi * shtns->mres)) * (ip)
@@ -112,6 +116,11 @@ int legendre_sphPlm_array(shtns_cfg shtns, const int lmax, const int im, const d
    (SHTns only promises nothing about it). */
 void spat_to_SH(shtns_cfg shtns, double *Vr, cplx *Qlm);
 void SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr);
+
+/* complex-valued scalar fields, mres = 1 (shtns.h:272, 277).  alm[LM_cplx(l, m)] holds
+   -l <= m <= l, nlm_cplx entries; z has the real transforms' layout with complex elements. */
+void SH_to_spat_cplx(shtns_cfg shtns, cplx *alm, cplx *z);
+void spat_cplx_to_SH(shtns_cfg shtns, cplx *z, cplx *alm);
 
 /* 2D vector transforms (shtns.h:283-289) */
 void spat_to_SHsphtor(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm);

@@ -143,3 +143,52 @@ class Oracle:
         t = np.empty(self.nlm, dtype=np.complex128)
         _load().orc_spat_to_SHsphtor(self._h, _p(a), _p(b), _p(s), _p(t), int(ltr))
         return s, t
+
+    # ---- complex-valued fields (mres = 1) ----
+    def lm_cplx(self, l, m):
+        """Index of (l, m), -l <= m <= l, in the complex-field spectral layout (shtns.h:114)."""
+        M = self.mmax
+        return l * (l + 1) + m if l <= M else M * (2 * l - M) + l + m
+
+    @property
+    def nlm_cplx(self):
+        return self.lm_cplx(self.lmax, min(self.lmax, self.mmax)) + 1
+
+    def _plm(self, m):
+        """P_lm(cos theta_k) for l = m..lmax at every grid latitude: float64[nlat, lmax-m+1]."""
+        return np.array([self.legendre(m, float(x), float(s)) for x, s in zip(self.cos_theta, self.sin_theta)])
+
+    def synth_cplx(self, alm):
+        """SH_to_spat_cplx (sht_func.c:1000-1052).  For each order m >= 0, the Legendre sums of
+        a_{l,m} go to Fourier bin m and those of (-1)^m a_{l,-m} to bin nphi - m; then a
+        backward (e^{+i m phi}, unnormalized) complex DFT along phi.
+        Returns complex128[nphi, nlat]."""
+        assert self.mres == 1
+        a = np.ascontiguousarray(alm, dtype=np.complex128)
+        assert a.shape == (self.nlm_cplx,)
+        G = np.zeros((self.nphi, self.nlat), dtype=np.complex128)
+        for m in range(self.mmax + 1):
+            P = self._plm(m)
+            ls = range(m, self.lmax + 1)
+            G[m] = P @ a[[self.lm_cplx(l, m) for l in ls]]
+            if m > 0:
+                G[self.nphi - m] = (-1) ** m * (P @ a[[self.lm_cplx(l, -m) for l in ls]])
+        return np.fft.ifft(G, axis=0) * self.nphi
+
+    def analys_cplx(self, z):
+        """spat_cplx_to_SH (sht_func.c:943-995).  DFT along phi (bin m = sum_j z_j e^{-i m phi_j}),
+        then per order the Gauss-weighted Legendre projections, scaled by 2 pi / nphi: bin m
+        gives a_{l,m}, bin nphi - m gives (-1)^m a_{l,-m}."""
+        assert self.mres == 1
+        z = np.ascontiguousarray(z, dtype=np.complex128)
+        assert z.shape == (self.nphi, self.nlat)
+        Z = np.fft.fft(z, axis=0) * (2 * np.pi / self.nphi)
+        w = self.gauss_w
+        a = np.zeros(self.nlm_cplx, dtype=np.complex128)
+        for m in range(self.mmax + 1):
+            P = self._plm(m)
+            ls = list(range(m, self.lmax + 1))
+            a[[self.lm_cplx(l, m) for l in ls]] = P.T @ (w * Z[m])
+            if m > 0:
+                a[[self.lm_cplx(l, -m) for l in ls]] = (-1) ** m * (P.T @ (w * Z[self.nphi - m]))
+        return a

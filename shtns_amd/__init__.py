@@ -75,6 +75,7 @@ class sht:
             raise RuntimeError("shtns_create failed")
         info = self._cfg.contents
         self.lmax, self.mmax, self.mres, self.nlm = info.lmax, info.mmax, info.mres, info.nlm
+        self.nlm_cplx = info.nlm_cplx
         self.nlat = self.nphi = self.nspat = 0
         self.l = np.zeros(self.nlm, dtype=np.int32)
         self.m = np.zeros(self.nlm, dtype=np.int32)
@@ -221,6 +222,78 @@ class sht:
         _lib.load().spat_to_SHqst_l(self._cfg, _ptr(Vr), _ptr(Vt), _ptr(Vp), _ptr(Qlm), _ptr(Slm), _ptr(Tlm),
                                     self.lmax if ltr < 0 else ltr)
         _lib.check_error("spat_to_SHqst")
+
+    def SH_to_spat_cplx(self, alm, z):
+        """Complex-valued field: alm[LM_cplx(l, m)], -l <= m <= l -> z (shtns_numpy.i:333-337)."""
+        self._need_grid()
+        _lib.load().SH_to_spat_cplx(self._cfg, _ptr(alm), _ptr(z))
+        _lib.check_error("SH_to_spat_cplx")
+
+    def spat_cplx_to_SH(self, z, alm):
+        """Complex-valued field z -> alm[LM_cplx(l, m)] (shtns_numpy.i:328-332)."""
+        self._need_grid()
+        _lib.load().spat_cplx_to_SH(self._cfg, _ptr(z), _ptr(alm))
+        _lib.check_error("spat_cplx_to_SH")
+
+    def zidx(self, l, m):
+        """Index of (l, m), -l <= m <= l, in a complex field's spectrum (LM_cplx, shtns.h:114)."""
+        if l > self.lmax or abs(m) > min(l, self.mmax):
+            raise ValueError("l or m invalid")
+        return l * (l + 1) + m if l <= self.mmax else self.mmax * (2 * l - self.mmax) + l + m
+
+    def _need_cplx(self, n):
+        self._need_grid()
+        if self.lmax != self.mmax or self.mres != 1:
+            raise RuntimeError("complex SH requires lmax=mmax and mres=1.")
+        if n != 1:
+            raise RuntimeError("complex-valued transforms: one array (scalar field) supported.")
+
+    def synth_cplx(self, *arg):
+        """z = synth_cplx(zlm): complex-valued scalar field from its (lmax+1)^2 coefficients
+        (shtns_numpy.i:469-500).  numpy arrays use the host API; a torch complex128 tensor on
+        the GPU stays on the device."""
+        self._need_cplx(len(arg))
+        a = arg[0]
+        if a.shape[-1] != self.nlm_cplx or len(a.shape) != 1:
+            raise RuntimeError("spectral array has wrong size.")
+        if _is_torch(a):
+            import torch
+            if not a.is_cuda or a.dtype != torch.complex128:
+                raise RuntimeError("spectral tensor should be torch.complex128 on the GPU")
+            a = a.contiguous()
+            z = torch.empty(tuple(self.spat_shape), dtype=torch.complex128, device=a.device)
+            self._stream(a)
+            _lib.load().cu_SH_to_spat_cplx(self._cfg, _ptr(a), _ptr(z))
+            _lib.check_error("synth_cplx")
+            return z
+        if a.dtype != np.complex128:
+            raise RuntimeError("spectral array should be dtype=complex.")
+        z = np.empty(self.spat_shape, dtype=np.complex128)
+        self.SH_to_spat_cplx(np.ascontiguousarray(a), z)
+        return z
+
+    def analys_cplx(self, *arg):
+        """zlm = analys_cplx(z): coefficients of a complex-valued scalar field
+        (shtns_numpy.i:502-535)."""
+        self._need_cplx(len(arg))
+        z = arg[0]
+        if tuple(z.shape) != tuple(self.spat_shape):
+            raise RuntimeError("spatial array has wrong shape.")
+        if _is_torch(z):
+            import torch
+            if not z.is_cuda or z.dtype != torch.complex128:
+                raise RuntimeError("spatial tensor should be torch.complex128 on the GPU")
+            z = z.contiguous()
+            a = torch.empty((self.nlm_cplx,), dtype=torch.complex128, device=z.device)
+            self._stream(z)
+            _lib.load().cu_spat_cplx_to_SH(self._cfg, _ptr(z), _ptr(a))
+            _lib.check_error("analys_cplx")
+            return a
+        if z.dtype != np.complex128:
+            raise RuntimeError("spatial array should be dtype=complex128.")
+        a = np.empty(self.nlm_cplx, dtype=np.complex128)
+        self.spat_cplx_to_SH(np.ascontiguousarray(z), a)
+        return a
 
     # ---- generic entry points (numpy or torch) ----
     def synth(self, *arg, ltr=-1):

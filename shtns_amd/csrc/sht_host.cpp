@@ -1250,3 +1250,68 @@ extern "C" void spat_to_SHqst(shtns_cfg c, double *Vr, double *Vt, double *Vp, c
     spat_to_SHqst_l(c, Vr, Vt, Vp, Q, S, T, c->lmax);
 }
 extern "C" void SHtor_to_spat(shtns_cfg c, cplx *T, double *Vt, double *Vp) { SHtor_to_spat_l(c, T, Vt, Vp, c->lmax); }
+
+/* Complex-valued scalar fields (mres = 1).  The field's real and imaginary parts are
+   transformed as one batch of two real fields; the spectra are split and merged between the
+   complex layout (LM_cplx, -l <= m <= l) and the two parts' real-field spectra by small
+   kernels.  SHTns transforms orders +m and -m on the CPU with one complex FFT
+   (sht_func.c:943-1052); the Legendre work is the same. */
+static bool cplx_ready(shtns_cfg c, const char *who) {
+    if (!ready(c, who)) return false;
+    if (c->mres != 1) runerr("complex SH requires mres=1.");
+    return true;
+}
+extern "C" void cu_SH_to_spat_cplx(shtns_cfg c, cplx *alm, cplx *z) {
+    clear_error();
+    if (!cplx_ready(c, "cu_SH_to_spat_cplx")) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
+    char *base = (char *)xscratch(c, qb + vb);
+    if (!base) return;
+    double2 *XY = (double2 *)base;
+    double *xy = (double *)(base + qb);
+    launch_cplx_split(reinterpret_cast<const double2 *>(alm), XY, XY + c->nlm, c->lmax, c->mmax, c->comp_stream);
+    synth_core(c, reinterpret_cast<const cplx *>(XY), xy, c->lmax, 2, c->nlm, c->nspat, 1.0, 0, prec_f64(c));
+    if (g_last_err) return;
+    launch_interleave(xy, reinterpret_cast<double2 *>(z), c->nspat, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+extern "C" void cu_spat_cplx_to_SH(shtns_cfg c, cplx *z, cplx *alm) {
+    clear_error();
+    if (!cplx_ready(c, "cu_spat_cplx_to_SH")) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
+    char *base = (char *)xscratch(c, qb + vb);
+    if (!base) return;
+    double2 *XY = (double2 *)base;
+    double *xy = (double *)(base + qb);
+    launch_deinterleave(reinterpret_cast<const double2 *>(z), xy, c->nspat, c->comp_stream);
+    anal_core(c, xy, reinterpret_cast<cplx *>(XY), c->lmax, 2, c->nspat, c->nlm, 1.0, 1, prec_f64(c));
+    if (g_last_err) return;
+    launch_cplx_merge(XY, XY + c->nlm, reinterpret_cast<double2 *>(alm), c->lmax, c->mmax, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+extern "C" void SH_to_spat_cplx(shtns_cfg c, cplx *alm, cplx *z) {
+    clear_error();
+    if (!cplx_ready(c, "SH_to_spat_cplx")) return;
+    HostStage st{c, {}};
+    const size_t ab = sizeof(cplx) * c->nlm_cplx, zb = sizeof(cplx) * c->nspat;
+    cplx *da = (cplx *)st.dev(ab), *dz = (cplx *)st.dev(zb);
+    if (!da || !dz) { set_error(2, "SH_to_spat_cplx: device allocation failed"); return; }
+    if (!h2d(da, alm, ab, c->comp_stream)) { set_error(2, "SH_to_spat_cplx: copy failed"); return; }
+    cu_SH_to_spat_cplx(c, da, dz);
+    if (g_last_err) return;
+    if (!d2h(z, dz, zb, c->comp_stream)) { set_error(2, "SH_to_spat_cplx: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(c->comp_stream));
+}
+extern "C" void spat_cplx_to_SH(shtns_cfg c, cplx *z, cplx *alm) {
+    clear_error();
+    if (!cplx_ready(c, "spat_cplx_to_SH")) return;
+    HostStage st{c, {}};
+    const size_t ab = sizeof(cplx) * c->nlm_cplx, zb = sizeof(cplx) * c->nspat;
+    cplx *da = (cplx *)st.dev(ab), *dz = (cplx *)st.dev(zb);
+    if (!da || !dz) { set_error(2, "spat_cplx_to_SH: device allocation failed"); return; }
+    if (!h2d(dz, z, zb, c->comp_stream)) { set_error(2, "spat_cplx_to_SH: copy failed"); return; }
+    cu_spat_cplx_to_SH(c, dz, da);
+    if (g_last_err) return;
+    if (!d2h(alm, da, ab, c->comp_stream)) { set_error(2, "spat_cplx_to_SH: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(c->comp_stream));
+}

@@ -172,6 +172,12 @@ void ensure_lds_limit(const void *kern, size_t lds);
 // element-wise FP32 <-> FP64 copies of nfields arrays of n scalars (sht_kernels.hip)
 void launch_convert(const float *src, long src_dist, double *dst, long dst_dist, long n, int nfields, hipStream_t s);
 void launch_convert(const double *src, long src_dist, float *dst, long dst_dist, long n, int nfields, hipStream_t s);
+// complex-valued fields: spectra split into / merged from the real and imaginary parts'
+// real-field spectra, and the spatial interleave of the two parts
+void launch_cplx_split(const double2 *a, double2 *X, double2 *Y, int lmax, int mmax, hipStream_t s);
+void launch_cplx_merge(const double2 *X, const double2 *Y, double2 *a, int lmax, int mmax, hipStream_t s);
+void launch_interleave(const double *xy, double2 *z, long n, hipStream_t s);
+void launch_deinterleave(const double2 *z, double *xy, long n, hipStream_t s);
 // the two halves of launch_legendre_synth: coefficient pre-mix, then the Legendre kernel for
 // latitude tiles [tile0, tile0 + ntl) (64 latitude pairs per tile)
 void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop, double *qmix,

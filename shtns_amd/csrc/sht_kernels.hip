@@ -222,6 +222,78 @@ void launch_convert(const double *src, long src_dist, float *dst, long dst_dist,
                        dst_dist, n);
 }
 
+// ---- complex-valued fields (mres = 1) ----
+// A complex field z = x + i y is transformed as its two real fields x and y, one batch of 2.
+// Spectral layouts: complex-field a[LM_cplx(l, m)], -l <= m <= l (shtns.h:114); real-field
+// X[LM(l, m)], m >= 0.  With Y_l^{-m} = (-1)^m conj(Y_l^m), for m > 0 and b = (-1)^m a_{l,-m}:
+//   X_lm = (a_lm + conj b) / 2,  Y_lm = (a_lm - conj b) / (2i);  m = 0: X = Re a, Y = Im a
+// and back: a_lm = X + i Y, a_{l,-m} = (-1)^m (conj X + i conj Y)  (sht_func.c:900-935).
+__device__ __forceinline__ long lm_real(int l, int m, int lmax) { return ((long)m * (2 * lmax + 1 - m)) / 2 + l; }
+__device__ __forceinline__ long lm_cplx(int l, int m, int mmax) {
+    return (l <= mmax) ? (long)l * (l + 1) + m : (long)mmax * (2 * l - mmax) + l + m;
+}
+
+__global__ void cplx_split_kernel(const double2 *__restrict__ a, double2 *__restrict__ X, double2 *__restrict__ Y,
+                                  int lmax, int mmax) {
+    const int m = blockIdx.x;
+    const double sgn = (m & 1) ? -1.0 : 1.0;
+    for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
+        const double2 ap = a[lm_cplx(l, m, mmax)];
+        const long lm = lm_real(l, m, lmax);
+        if (m == 0) {
+            X[lm] = make_double2(ap.x, 0.0);
+            Y[lm] = make_double2(ap.y, 0.0);
+        } else {
+            const double2 an = a[lm_cplx(l, -m, mmax)];
+            const double bx = sgn * an.x, by = sgn * an.y;
+            X[lm] = make_double2(0.5 * (ap.x + bx), 0.5 * (ap.y - by));
+            Y[lm] = make_double2(0.5 * (ap.y + by), 0.5 * (bx - ap.x));
+        }
+    }
+}
+
+__global__ void cplx_merge_kernel(const double2 *__restrict__ X, const double2 *__restrict__ Y, double2 *__restrict__ a,
+                                  int lmax, int mmax) {
+    const int m = blockIdx.x;
+    const double sgn = (m & 1) ? -1.0 : 1.0;
+    for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
+        const long lm = lm_real(l, m, lmax);
+        const double2 x = X[lm], y = Y[lm];
+        a[lm_cplx(l, m, mmax)] = make_double2(x.x - y.y, x.y + y.x);
+        if (m > 0) a[lm_cplx(l, -m, mmax)] = make_double2(sgn * (x.x + y.y), sgn * (y.x - x.y));
+    }
+}
+
+// z[i] = (x[i], y[i]) and back; x and y are one batch of two real fields, field distance n
+__global__ void interleave_kernel(const double *__restrict__ xy, double2 *__restrict__ z, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        z[i] = make_double2(xy[i], xy[n + i]);
+}
+__global__ void deinterleave_kernel(const double2 *__restrict__ z, double *__restrict__ xy, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double2 v = z[i];
+        xy[i] = v.x;
+        xy[n + i] = v.y;
+    }
+}
+
+void launch_cplx_split(const double2 *a, double2 *X, double2 *Y, int lmax, int mmax, hipStream_t s) {
+    hipLaunchKernelGGL(cplx_split_kernel, dim3(mmax + 1), dim3(256), 0, s, a, X, Y, lmax, mmax);
+}
+void launch_cplx_merge(const double2 *X, const double2 *Y, double2 *a, int lmax, int mmax, hipStream_t s) {
+    hipLaunchKernelGGL(cplx_merge_kernel, dim3(mmax + 1), dim3(256), 0, s, X, Y, a, lmax, mmax);
+}
+void launch_interleave(const double *xy, double2 *z, long n, hipStream_t s) {
+    if (n <= 0) return;
+    const int nb = (int)std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(interleave_kernel, dim3(nb), dim3(256), 0, s, xy, z, n);
+}
+void launch_deinterleave(const double2 *z, double *xy, long n, hipStream_t s) {
+    if (n <= 0) return;
+    const int nb = (int)std::min<long>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(deinterleave_kernel, dim3(nb), dim3(256), 0, s, z, xy, n);
+}
+
 void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
                            double2 *F, long four_dist, double *qmix, int nsteps_total, hipStream_t s) {
     launch_premix(a, Qlm, spec_dist, spec_ltop, qmix, nsteps_total, s);

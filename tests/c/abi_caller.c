@@ -16,7 +16,8 @@ int main(void) {
     legendre_sphPlm_array(c, 31, 2, 0.3, yl);
     printf("nlm=%u lmax=%u mmax=%u mres=%u nlat=%u nlat_2=%u nphi=%u nspat=%u ret=%d\n", c->nlm, c->lmax, c->mmax,
            c->mres, c->nlat, c->nlat_2, c->nphi, c->nspat, nspat);
-    printf("LM11=%d LiM=%d nlm_calc=%ld nlm_cplx=%u\n", (int)LM(c, 1, 1), (int)LiM(c, 5, 3), nlm_calc(31, 31, 1), c->nlm_cplx);
+    printf("LM11=%d LiM=%d nlm_calc=%ld nlm_cplx=%u LMc=%d\n", (int)LM(c, 1, 1), (int)LiM(c, 5, 3), nlm_calc(31, 31, 1),
+           c->nlm_cplx, (int)LM_cplx(c, 5, -3));
     printf("ct0=%.17g st0=%.17g li5=%u mi40=%u nw=%d w0=%.17g y2=%.17g\n", c->ct[0], c->st[0], c->li[5], c->mi[40], nw,
            w[0], yl[0]);
     printf("sh00=%.17g sh10=%.17g sh11=%.17g\n", sh00_1(c), sh10_ct(c), sh11_st(c));

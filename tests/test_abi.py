@@ -51,6 +51,7 @@ def test_c_caller_against_public_header(tmp_path):
     assert int(kv["nspat"]) == int(kv["ret"]) == 32 * 64
     assert int(kv["LM11"]) == 32 and int(kv["LiM"]) == o.idx(5, 3)
     assert int(kv["nlm_calc"]) == 528 and int(kv["nlm_cplx"]) == 2 * 528 - 32
+    assert int(kv["LMc"]) == o.lm_cplx(5, -3) == 27
     assert abs(float(kv["ct0"]) - o.cos_theta[0]) < 2e-16 and abs(float(kv["st0"]) - o.sin_theta[0]) < 1e-15
     assert int(kv["li5"]) == 5 and int(kv["mi40"]) == 1
     assert int(kv["nw"]) == 16 and abs(float(kv["w0"]) - o.gauss_w[0]) < 1e-14 * o.gauss_w[0]

@@ -1,0 +1,120 @@
+"""GPU: complex-valued scalar transforms SH_to_spat_cplx / spat_cplx_to_SH (shtns.h:272, 277;
+sht_func.c:943-1052), their device forms cu_SH_to_spat_cplx / cu_spat_cplx_to_SH and the
+Python synth_cplx / analys_cplx (shtns_numpy.i:469-535), against the oracle's complex
+transforms (pinned against scipy's complex spherical harmonics on the CPU)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import shtns_amd
+from shtns_amd import _lib
+from shtns_amd.synthetic import spectral_field
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((np.asarray(a) - b).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
+
+
+def make(lmax, mmax, nlat, nphi, layout=shtns_amd.SHT_THETA_CONTIGUOUS):
+    s = shtns_amd.sht(lmax, mmax)
+    s.set_grid(nlat, nphi, flags=layout | shtns_amd.sht_gauss, polar_opt=0.0)
+    return s, Oracle(lmax, mmax, 1, nlat, nphi)
+
+
+def random_alm(n, seed, lmax, o):
+    """complex coefficients with the spectra's usual decay: |a_lm| ~ 1/(l+1)"""
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for l in range(lmax + 1):
+        for m in range(-min(l, o.mmax), min(l, o.mmax) + 1):
+            a[o.lm_cplx(l, m)] /= (l + 1)
+    return a
+
+
+@pytest.mark.parametrize("case", [(31, 32, 64), (63, 96, 192), (127, 128, 256)])
+def test_cplx_matches_oracle(case, device):
+    L, nlat, nphi = case
+    s, o = make(L, L, nlat, nphi)
+    assert s.nlm_cplx == (L + 1) ** 2 == o.nlm_cplx
+    a = random_alm(s.nlm_cplx, 70, L, o)
+    z = s.synth_cplx(a)
+    assert z.shape == (nphi, nlat) and z.dtype == np.complex128
+    assert rel_l2(z, o.synth_cplx(a)) <= TOL
+    # analysis of a field that is not band-limited
+    rng = np.random.default_rng(71)
+    w = rng.standard_normal((nphi, nlat)) + 1j * rng.standard_normal((nphi, nlat))
+    assert rel_l2(s.analys_cplx(w), o.analys_cplx(w)) <= TOL
+    # band-limited round trip
+    assert rel_l2(s.analys_cplx(z), a) <= 1e-13
+
+
+def test_cplx_phi_contiguous(device):
+    s, o = make(47, 47, 48, 96, layout=shtns_amd.SHT_PHI_CONTIGUOUS)
+    assert s.spat_shape == (48, 96)
+    a = random_alm(s.nlm_cplx, 72, 47, o)
+    z = s.synth_cplx(a)
+    assert rel_l2(z, o.synth_cplx(a).T) <= TOL
+    assert rel_l2(s.analys_cplx(z), a) <= 1e-13
+
+
+def test_cplx_mmax_below_lmax_c_api(device):
+    """The C functions take mmax < lmax (nlm_cplx_calc layout); the Python methods, like
+    SHTns's, require lmax = mmax."""
+    s, o = make(40, 20, 64, 48)
+    assert s.nlm_cplx == o.nlm_cplx
+    a = random_alm(s.nlm_cplx, 73, 40, o)
+    z = np.empty((48, 64), dtype=np.complex128)
+    lib = _lib.load()
+    lib.SH_to_spat_cplx(s._cfg, a.ctypes.data_as(ctypes.c_void_p), z.ctypes.data_as(ctypes.c_void_p))
+    assert lib.shtns_last_error() == 0
+    assert rel_l2(z, o.synth_cplx(a)) <= TOL
+    a2 = np.empty_like(a)
+    lib.spat_cplx_to_SH(s._cfg, z.ctypes.data_as(ctypes.c_void_p), a2.ctypes.data_as(ctypes.c_void_p))
+    assert rel_l2(a2, a) <= 1e-13
+    with pytest.raises(RuntimeError):
+        s.synth_cplx(a)
+
+
+def test_cplx_real_field_equals_real_transform(device):
+    """For a real field the m >= 0 coefficients are the real transform's, bit for bit (the
+    real part is analysed in the same kernels as one field of a batch), and the m < 0 ones
+    (-1)^m conj of them."""
+    s, o = make(63, 63, 96, 192)
+    q = spectral_field(63, 63, seed=74)
+    v = s.synth(q)
+    a = s.analys_cplx(v.astype(np.complex128))
+    q2 = s.analys(v)
+    for l in range(64):
+        for m in range(l + 1):
+            assert a[s.zidx(l, m)] == q2[s.idx(l, m)], (l, m)
+            assert abs(a[s.zidx(l, -m)] - (-1) ** m * np.conj(q2[s.idx(l, m)])) <= 1e-15 * (1 + abs(q2[s.idx(l, m)]))
+    z = s.synth_cplx(a)
+    assert np.max(np.abs(z.imag)) <= 1e-12 * np.max(np.abs(v)) and rel_l2(z.real, v) <= 1e-13
+
+
+def test_cplx_torch_equals_host(device):
+    import torch
+    s, o = make(95, 95, 96, 192)
+    a = random_alm(s.nlm_cplx, 75, 95, o)
+    z = s.synth_cplx(a)
+    zt = s.synth_cplx(torch.from_numpy(a).to(device))
+    assert zt.dtype == torch.complex128 and tuple(zt.shape) == (192, 96)
+    assert np.array_equal(zt.cpu().numpy(), z)
+    at = s.analys_cplx(zt)
+    assert np.array_equal(at.cpu().numpy(), s.analys_cplx(z))
+
+
+def test_cplx_rejects(device):
+    s, _ = make(31, 31, 32, 64)
+    a = np.zeros(s.nlm_cplx, dtype=np.complex128)
+    with pytest.raises(RuntimeError):
+        s.synth_cplx(a, a)
+    with pytest.raises(RuntimeError):
+        s.synth_cplx(np.zeros(s.nlm, dtype=np.complex128))
+    with pytest.raises(RuntimeError):
+        s.analys_cplx(np.zeros(s.spat_shape))    # float64, not complex
