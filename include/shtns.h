@@ -8,7 +8,8 @@
  * Scope: Gauss and regular (Fejer) grids; the scalar transforms SH_to_spat/spat_to_SH, the
  * 2D vector transforms SHsphtor_to_spat/spat_to_SHsphtor and the 3-component transforms
  * SHqst_to_spat/spat_to_SHqst (plus their _l truncated forms) and the complex-valued
- * scalar transforms SH_to_spat_cplx/spat_cplx_to_SH, computed on the GPU by
+ * scalar and 2D vector transforms (SH_to_spat_cplx, SHsphtor_to_spat_cplx and their
+ * inverses), computed on the GPU by
  * hand-written HIP kernels.  Host-pointer transforms copy to the device,
  * run there and copy back; the device-pointer variants are in shtns_hip.h.
  */
@@ -121,6 +122,13 @@ void SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr);
    -l <= m <= l, nlm_cplx entries; z has the real transforms' layout with complex elements. */
 void SH_to_spat_cplx(shtns_cfg shtns, cplx *alm, cplx *z);
 void spat_cplx_to_SH(shtns_cfg shtns, cplx *z, cplx *alm);
+/* complex-valued 2D vector fields, mres = 1 (shtns.h:292, 294): Slm, Tlm in the LM_cplx
+   layout; Vt, Vp complex with the real transforms' layout */
+void spat_cplx_to_SHsphtor(shtns_cfg, cplx *Vt, cplx *Vp, cplx *Slm, cplx *Tlm);
+void SHsphtor_to_spat_cplx(shtns_cfg, cplx *Slm, cplx *Tlm, cplx *Vt, cplx *Vp);
+/* complex-valued 3-component fields (shtns.h:308-309) */
+void spat_cplx_to_SHqst(shtns_cfg, cplx *Vr, cplx *Vt, cplx *Vp, cplx *Qlm, cplx *Slm, cplx *Tlm);
+void SHqst_to_spat_cplx(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, cplx *Vr, cplx *Vt, cplx *Vp);
 
 /* 2D vector transforms (shtns.h:283-289) */
 void spat_to_SHsphtor(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm);

@@ -32,11 +32,14 @@ void cu_spat_to_SHqst(shtns_cfg, double *Vr, double *Vt, double *Vp, cplx *Qlm, 
 void cu_SHqst_to_spat(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double *Vr, double *Vt, double *Vp,
                       int ltr);                                                             /* :52 */
 
-/* Complex-valued scalar fields on device arrays (additive: SHTns's complex transforms,
- * sht_func.c:943-1052, run on the CPU only).  Same layouts as SH_to_spat_cplx /
- * spat_cplx_to_SH; asynchronous on the config's compute stream; mres = 1. */
+/* Complex-valued scalar and 2D vector fields on device arrays (additive: SHTns's complex
+ * transforms, sht_func.c:943-1200, run on the CPU only).  Same layouts as SH_to_spat_cplx /
+ * SHsphtor_to_spat_cplx and their inverses; asynchronous on the config's compute stream;
+ * mres = 1. */
 void cu_SH_to_spat_cplx(shtns_cfg, cplx *alm, cplx *z);
 void cu_spat_cplx_to_SH(shtns_cfg, cplx *z, cplx *alm);
+void cu_SHsphtor_to_spat_cplx(shtns_cfg, cplx *Slm, cplx *Tlm, cplx *Vt, cplx *Vp);
+void cu_spat_cplx_to_SHsphtor(shtns_cfg, cplx *Vt, cplx *Vp, cplx *Slm, cplx *Tlm);
 
 /* ---- initialization (shtns_cuda.h:63-72, sht_gpu.cu:227-336) ---- */
 shtns_cfg cushtns_clone(shtns_cfg shtns, hipStream_t compute_stream, hipStream_t transfer_stream); /* :63 */

@@ -192,3 +192,57 @@ class Oracle:
             if m > 0:
                 a[[self.lm_cplx(l, -m) for l in ls]] = (-1) ** m * (P.T @ (w * Z[self.nphi - m]))
         return a
+
+    def _plm_dtheta(self, m):
+        """P_lm and dP_lm/dtheta at every latitude, l = m..lmax (two float64[nlat, lmax-m+1]):
+        sin(theta) dP_l/dtheta = l cos(theta) P_l - sqrt((2l+1)/(2l-1) (l^2 - m^2)) P_{l-1}."""
+        P = self._plm(m)
+        ls = np.arange(m, self.lmax + 1, dtype=np.float64)
+        Pm1 = np.zeros_like(P)
+        Pm1[:, 1:] = P[:, :-1]
+        c = np.sqrt(np.abs((2 * ls + 1) / (2 * ls - 1) * (ls * ls - m * m)))
+        dP = (ls * self.cos_theta[:, None] * P - c * Pm1) / self.sin_theta[:, None]
+        return P, dP
+
+    def synth_sphtor_cplx(self, slm, tlm):
+        """SHsphtor_to_spat_cplx (sht_func.c:1060-1126): per order, bins m and nphi - m from
+        orders +m and -m (coefficients times (-1)^m for -m), then a backward complex DFT.
+        Vt = dS/dtheta + (1/sin) dT/dphi, Vp = (1/sin) dS/dphi - dT/dtheta, d/dphi = i m."""
+        assert self.mres == 1
+        s = np.ascontiguousarray(slm, dtype=np.complex128)
+        t = np.ascontiguousarray(tlm, dtype=np.complex128)
+        Gt = np.zeros((self.nphi, self.nlat), dtype=np.complex128)
+        Gp = np.zeros_like(Gt)
+        for m in range(self.mmax + 1):
+            P, dP = self._plm_dtheta(m)
+            ls = range(m, self.lmax + 1)
+            for mm, row, sign in ([(0, 0, 1.0)] if m == 0 else [(m, m, 1.0), (-m, self.nphi - m, (-1.0) ** m)]):
+                sl = sign * s[[self.lm_cplx(l, mm) for l in ls]]
+                tl = sign * t[[self.lm_cplx(l, mm) for l in ls]]
+                imP = (1j * mm) * P / self.sin_theta[:, None]
+                Gt[row] = dP @ sl + imP @ tl
+                Gp[row] = imP @ sl - dP @ tl
+        return np.fft.ifft(Gt, axis=0) * self.nphi, np.fft.ifft(Gp, axis=0) * self.nphi
+
+    def analys_sphtor_cplx(self, zt, zp):
+        """spat_cplx_to_SHsphtor (sht_func.c:1133-1200): DFT along phi, then per order the
+        Gauss-quadrature projections on grad Y and r x grad Y, divided by l(l+1); bin m gives
+        order +m, bin nphi - m order -m (times (-1)^m).  S_00 = T_00 = 0."""
+        assert self.mres == 1
+        Zt = np.fft.fft(np.ascontiguousarray(zt, dtype=np.complex128), axis=0) * (2 * np.pi / self.nphi)
+        Zp = np.fft.fft(np.ascontiguousarray(zp, dtype=np.complex128), axis=0) * (2 * np.pi / self.nphi)
+        w = self.gauss_w
+        s = np.zeros(self.nlm_cplx, dtype=np.complex128)
+        t = np.zeros_like(s)
+        for m in range(self.mmax + 1):
+            P, dP = self._plm_dtheta(m)
+            ls = np.arange(m, self.lmax + 1)
+            inv = np.zeros(len(ls))
+            inv[ls > 0] = 1.0 / (ls[ls > 0] * (ls[ls > 0] + 1.0))
+            for mm, row, sign in ([(0, 0, 1.0)] if m == 0 else [(m, m, 1.0), (-m, self.nphi - m, (-1.0) ** m)]):
+                a, b = w * Zt[row], w * Zp[row]
+                imP = (-1j * mm) * P / self.sin_theta[:, None]     # (1/sin) d/dphi of conj(Y)
+                idx = [self.lm_cplx(l, mm) for l in ls]
+                s[idx] = sign * inv * (a @ dP + b @ imP)
+                t[idx] = sign * inv * (a @ imP - b @ dP)
+        return s, t

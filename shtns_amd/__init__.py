@@ -241,59 +241,100 @@ class sht:
             raise ValueError("l or m invalid")
         return l * (l + 1) + m if l <= self.mmax else self.mmax * (2 * l - self.mmax) + l + m
 
+    def SHsphtor_to_spat_cplx(self, Slm, Tlm, Vt, Vp):
+        """Complex-valued 2D vector field from Slm, Tlm in the LM_cplx layout (shtns_numpy.i:365-369)."""
+        self._need_grid()
+        _lib.load().SHsphtor_to_spat_cplx(self._cfg, _ptr(Slm), _ptr(Tlm), _ptr(Vt), _ptr(Vp))
+        _lib.check_error("SHsphtor_to_spat_cplx")
+
+    def spat_cplx_to_SHsphtor(self, Vt, Vp, Slm, Tlm):
+        """Complex-valued 2D vector field -> Slm, Tlm (shtns_numpy.i:360-364)."""
+        self._need_grid()
+        _lib.load().spat_cplx_to_SHsphtor(self._cfg, _ptr(Vt), _ptr(Vp), _ptr(Slm), _ptr(Tlm))
+        _lib.check_error("spat_cplx_to_SHsphtor")
+
     def _need_cplx(self, n):
         self._need_grid()
         if self.lmax != self.mmax or self.mres != 1:
             raise RuntimeError("complex SH requires lmax=mmax and mres=1.")
-        if n != 1:
-            raise RuntimeError("complex-valued transforms: one array (scalar field) supported.")
+        if n not in (1, 2, 3):
+            raise RuntimeError("1, 2 or 3 arrays required.")
 
     def synth_cplx(self, *arg):
-        """z = synth_cplx(zlm): complex-valued scalar field from its (lmax+1)^2 coefficients
-        (shtns_numpy.i:469-500).  numpy arrays use the host API; a torch complex128 tensor on
-        the GPU stays on the device."""
-        self._need_cplx(len(arg))
-        a = arg[0]
-        if a.shape[-1] != self.nlm_cplx or len(a.shape) != 1:
-            raise RuntimeError("spectral array has wrong size.")
-        if _is_torch(a):
+        """z = synth_cplx(zlm) ; zt, zp = synth_cplx(slm, tlm) ; zr, zt, zp = synth_cplx(qlm, slm, tlm):
+        complex-valued fields from their (lmax+1)^2 coefficients per array
+        (shtns_numpy.i:469-500).  numpy arrays use the host API; torch complex128 tensors on
+        the GPU stay on the device."""
+        n = len(arg)
+        self._need_cplx(n)
+        for a in arg:
+            if len(a.shape) != 1 or a.shape[0] != self.nlm_cplx:
+                raise RuntimeError("spectral array has wrong size.")
+        if _is_torch(arg[0]):
             import torch
-            if not a.is_cuda or a.dtype != torch.complex128:
-                raise RuntimeError("spectral tensor should be torch.complex128 on the GPU")
-            a = a.contiguous()
-            z = torch.empty(tuple(self.spat_shape), dtype=torch.complex128, device=a.device)
-            self._stream(a)
-            _lib.load().cu_SH_to_spat_cplx(self._cfg, _ptr(a), _ptr(z))
-            _lib.check_error("synth_cplx")
-            return z
-        if a.dtype != np.complex128:
-            raise RuntimeError("spectral array should be dtype=complex.")
-        z = np.empty(self.spat_shape, dtype=np.complex128)
-        self.SH_to_spat_cplx(np.ascontiguousarray(a), z)
-        return z
+            q = []
+            for a in arg:
+                if not _is_torch(a) or not a.is_cuda or a.dtype != torch.complex128:
+                    raise RuntimeError("spectral tensors should be torch.complex128 on the GPU")
+                q.append(a.contiguous())
+            out = [torch.empty(tuple(self.spat_shape), dtype=torch.complex128, device=q[0].device) for _ in q]
+            self._stream(q[0])
+            lib = _lib.load()
+            if n != 2:
+                lib.cu_SH_to_spat_cplx(self._cfg, _ptr(q[0]), _ptr(out[0]))
+                _lib.check_error("synth_cplx")
+            if n >= 2:
+                lib.cu_SHsphtor_to_spat_cplx(self._cfg, _ptr(q[-2]), _ptr(q[-1]), _ptr(out[-2]), _ptr(out[-1]))
+                _lib.check_error("synth_cplx")
+            return out[0] if n == 1 else tuple(out)
+        q = []
+        for a in arg:
+            if a.dtype != np.complex128:
+                raise RuntimeError("spectral array should be dtype=complex.")
+            q.append(np.ascontiguousarray(a))
+        out = [np.empty(self.spat_shape, dtype=np.complex128) for _ in q]
+        if n != 2:
+            self.SH_to_spat_cplx(q[0], out[0])
+        if n >= 2:
+            self.SHsphtor_to_spat_cplx(q[-2], q[-1], out[-2], out[-1])
+        return out[0] if n == 1 else tuple(out)
 
     def analys_cplx(self, *arg):
-        """zlm = analys_cplx(z): coefficients of a complex-valued scalar field
+        """zlm = analys_cplx(z) ; slm, tlm = analys_cplx(zt, zp) ; qlm, slm, tlm = analys_cplx(zr, zt, zp)
         (shtns_numpy.i:502-535)."""
-        self._need_cplx(len(arg))
-        z = arg[0]
-        if tuple(z.shape) != tuple(self.spat_shape):
-            raise RuntimeError("spatial array has wrong shape.")
-        if _is_torch(z):
+        n = len(arg)
+        self._need_cplx(n)
+        for z in arg:
+            if tuple(z.shape) != tuple(self.spat_shape):
+                raise RuntimeError("spatial array has wrong shape.")
+        if _is_torch(arg[0]):
             import torch
-            if not z.is_cuda or z.dtype != torch.complex128:
-                raise RuntimeError("spatial tensor should be torch.complex128 on the GPU")
-            z = z.contiguous()
-            a = torch.empty((self.nlm_cplx,), dtype=torch.complex128, device=z.device)
-            self._stream(z)
-            _lib.load().cu_spat_cplx_to_SH(self._cfg, _ptr(z), _ptr(a))
-            _lib.check_error("analys_cplx")
-            return a
-        if z.dtype != np.complex128:
-            raise RuntimeError("spatial array should be dtype=complex128.")
-        a = np.empty(self.nlm_cplx, dtype=np.complex128)
-        self.spat_cplx_to_SH(np.ascontiguousarray(z), a)
-        return a
+            v = []
+            for z in arg:
+                if not _is_torch(z) or not z.is_cuda or z.dtype != torch.complex128:
+                    raise RuntimeError("spatial tensors should be torch.complex128 on the GPU")
+                v.append(z.contiguous())
+            out = [torch.empty((self.nlm_cplx,), dtype=torch.complex128, device=v[0].device) for _ in v]
+            self._stream(v[0])
+            lib = _lib.load()
+            if n != 2:
+                lib.cu_spat_cplx_to_SH(self._cfg, _ptr(v[0]), _ptr(out[0]))
+                _lib.check_error("analys_cplx")
+            if n >= 2:
+                lib.cu_spat_cplx_to_SHsphtor(self._cfg, _ptr(v[-2]), _ptr(v[-1]), _ptr(out[-2]), _ptr(out[-1]))
+                _lib.check_error("analys_cplx")
+            return out[0] if n == 1 else tuple(out)
+        v = []
+        for z in arg:
+            if z.dtype != np.complex128:
+                raise RuntimeError("spatial array should be dtype=complex128.")
+            v.append(np.ascontiguousarray(z))
+        out = [np.empty(self.nlm_cplx, dtype=np.complex128) for _ in v]
+        if n != 2:
+            self.spat_cplx_to_SH(v[0], out[0])
+        if n >= 2:
+            self.spat_cplx_to_SHsphtor(v[-2], v[-1], out[-2], out[-1])
+        return out[0] if n == 1 else tuple(out)
 
     # ---- generic entry points (numpy or torch) ----
     def synth(self, *arg, ltr=-1):

@@ -18,11 +18,13 @@ EXPORTED_SYMBOLS = [
     "SHsph_to_spat", "SHtor_to_spat", "spat_to_SH_l", "SH_to_spat_l", "SHsphtor_to_spat_l",
     "SHsph_to_spat_l", "SHtor_to_spat_l", "spat_to_SHsphtor_l", "SH_to_point",
     "spat_to_SHqst", "SHqst_to_spat", "spat_to_SHqst_l", "SHqst_to_spat_l",
-    "SH_to_spat_cplx", "spat_cplx_to_SH",
+    "SH_to_spat_cplx", "spat_cplx_to_SH", "SHsphtor_to_spat_cplx", "spat_cplx_to_SHsphtor",
+    "SHqst_to_spat_cplx", "spat_cplx_to_SHqst",
     "shtns_last_error", "shtns_last_error_string",
     # shtns_hip.h
     "cu_spat_to_SH", "cu_SH_to_spat", "cu_spat_to_SHsphtor", "cu_SHsphtor_to_spat",
     "cu_spat_to_SHqst", "cu_SHqst_to_spat", "cu_SH_to_spat_cplx", "cu_spat_cplx_to_SH",
+    "cu_SHsphtor_to_spat_cplx", "cu_spat_cplx_to_SHsphtor",
     "cushtns_clone", "cushtns_init_gpu", "cushtns_set_streams", "cushtns_release_gpu",
     "cu_SH_to_spat_batch", "cu_spat_to_SH_batch", "cu_SHsphtor_to_spat_batch",
     "cu_spat_to_SHsphtor_batch", "cu_SH_to_spat_adjoint_batch", "cu_spat_to_SH_adjoint_batch",
@@ -118,6 +120,12 @@ def load():
         "spat_cplx_to_SH": (None, [CFG, P, P]),
         "cu_SH_to_spat_cplx": (None, [CFG, P, P]),
         "cu_spat_cplx_to_SH": (None, [CFG, P, P]),
+        "SHsphtor_to_spat_cplx": (None, [CFG, P, P, P, P]),
+        "SHqst_to_spat_cplx": (None, [CFG, P, P, P, P, P, P]),
+        "spat_cplx_to_SHqst": (None, [CFG, P, P, P, P, P, P]),
+        "spat_cplx_to_SHsphtor": (None, [CFG, P, P, P, P]),
+        "cu_SHsphtor_to_spat_cplx": (None, [CFG, P, P, P, P]),
+        "cu_spat_cplx_to_SHsphtor": (None, [CFG, P, P, P, P]),
         "shtns_last_error": (I, []),
         "shtns_last_error_string": (ctypes.c_char_p, []),
         "cu_SH_to_spat": (None, [CFG, P, P, I]),

@@ -1289,6 +1289,83 @@ extern "C" void cu_spat_cplx_to_SH(shtns_cfg c, cplx *z, cplx *alm) {
     launch_cplx_merge(XY, XY + c->nlm, reinterpret_cast<double2 *>(alm), c->lmax, c->mmax, c->comp_stream);
     HIP_OK(hipGetLastError());
 }
+/* Complex-valued 2D vector fields: S and T are each split into the spectra of the real and
+   imaginary parts' vector fields; both parts go through the vector kernels as one batch of
+   two (SHTns: orders +m and -m per order, sht_func.c:1060-1200). */
+extern "C" void cu_SHsphtor_to_spat_cplx(shtns_cfg c, cplx *slm, cplx *tlm, cplx *zt, cplx *zp) {
+    clear_error();
+    if (!cplx_ready(c, "cu_SHsphtor_to_spat_cplx")) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
+    char *base = (char *)xscratch(c, 2 * qb + 2 * vb);
+    if (!base) return;
+    double2 *SS = (double2 *)base, *TT = (double2 *)(base + qb);
+    double *VT = (double *)(base + 2 * qb), *VP = (double *)(base + 2 * qb + vb);
+    launch_cplx_split(reinterpret_cast<const double2 *>(slm), SS, SS + c->nlm, c->lmax, c->mmax, c->comp_stream);
+    launch_cplx_split(reinterpret_cast<const double2 *>(tlm), TT, TT + c->nlm, c->lmax, c->mmax, c->comp_stream);
+    vsynth_core(c, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), VT, VP, c->lmax, 2, c->nlm, c->nspat);
+    if (g_last_err) return;
+    launch_interleave(VT, reinterpret_cast<double2 *>(zt), c->nspat, c->comp_stream);
+    launch_interleave(VP, reinterpret_cast<double2 *>(zp), c->nspat, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+extern "C" void cu_spat_cplx_to_SHsphtor(shtns_cfg c, cplx *zt, cplx *zp, cplx *slm, cplx *tlm) {
+    clear_error();
+    if (!cplx_ready(c, "cu_spat_cplx_to_SHsphtor")) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
+    char *base = (char *)xscratch(c, 2 * qb + 2 * vb);
+    if (!base) return;
+    double2 *SS = (double2 *)base, *TT = (double2 *)(base + qb);
+    double *VT = (double *)(base + 2 * qb), *VP = (double *)(base + 2 * qb + vb);
+    launch_deinterleave(reinterpret_cast<const double2 *>(zt), VT, c->nspat, c->comp_stream);
+    launch_deinterleave(reinterpret_cast<const double2 *>(zp), VP, c->nspat, c->comp_stream);
+    vanal_core(c, VT, VP, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), c->lmax, 2, c->nspat, c->nlm);
+    if (g_last_err) return;
+    launch_cplx_merge(SS, SS + c->nlm, reinterpret_cast<double2 *>(slm), c->lmax, c->mmax, c->comp_stream);
+    launch_cplx_merge(TT, TT + c->nlm, reinterpret_cast<double2 *>(tlm), c->lmax, c->mmax, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+extern "C" void SHsphtor_to_spat_cplx(shtns_cfg c, cplx *slm, cplx *tlm, cplx *zt, cplx *zp) {
+    clear_error();
+    if (!cplx_ready(c, "SHsphtor_to_spat_cplx")) return;
+    HostStage st{c, {}};
+    const size_t ab = sizeof(cplx) * c->nlm_cplx, zb = sizeof(cplx) * c->nspat;
+    cplx *ds = (cplx *)st.dev(ab), *dt = (cplx *)st.dev(ab), *dzt = (cplx *)st.dev(zb), *dzp = (cplx *)st.dev(zb);
+    if (!ds || !dt || !dzt || !dzp) { set_error(2, "SHsphtor_to_spat_cplx: device allocation failed"); return; }
+    if (!h2d(ds, slm, ab, c->comp_stream) || !h2d(dt, tlm, ab, c->comp_stream)) {
+        set_error(2, "SHsphtor_to_spat_cplx: copy failed");
+        return;
+    }
+    cu_SHsphtor_to_spat_cplx(c, ds, dt, dzt, dzp);
+    if (g_last_err) return;
+    if (!d2h(zt, dzt, zb, c->comp_stream) || !d2h(zp, dzp, zb, c->comp_stream)) {
+        set_error(2, "SHsphtor_to_spat_cplx: copy failed");
+        return;
+    }
+    HIP_OK(hipStreamSynchronize(c->comp_stream));
+}
+extern "C" void spat_cplx_to_SHsphtor(shtns_cfg c, cplx *zt, cplx *zp, cplx *slm, cplx *tlm) {
+    clear_error();
+    if (!cplx_ready(c, "spat_cplx_to_SHsphtor")) return;
+    HostStage st{c, {}};
+    const size_t ab = sizeof(cplx) * c->nlm_cplx, zb = sizeof(cplx) * c->nspat;
+    cplx *ds = (cplx *)st.dev(ab), *dt = (cplx *)st.dev(ab), *dzt = (cplx *)st.dev(zb), *dzp = (cplx *)st.dev(zb);
+    if (!ds || !dt || !dzt || !dzp) { set_error(2, "spat_cplx_to_SHsphtor: device allocation failed"); return; }
+    if (!h2d(dzt, zt, zb, c->comp_stream) || !h2d(dzp, zp, zb, c->comp_stream)) {
+        set_error(2, "spat_cplx_to_SHsphtor: copy failed");
+        return;
+    }
+    cu_spat_cplx_to_SHsphtor(c, dzt, dzp, ds, dt);
+    if (g_last_err) return;
+    if (!d2h(slm, ds, ab, c->comp_stream) || !d2h(tlm, dt, ab, c->comp_stream)) {
+        set_error(2, "spat_cplx_to_SHsphtor: copy failed");
+        return;
+    }
+    HIP_OK(hipStreamSynchronize(c->comp_stream));
+}
+/* complex-valued 3-component fields: the scalar then the vector transform
+   (sht_func.c:1197-1210) */
+extern "C" void SHqst_to_spat_cplx(shtns_cfg c, cplx *qlm, cplx *slm, cplx *tlm, cplx *zr, cplx *zt, cplx *zp);
+extern "C" void spat_cplx_to_SHqst(shtns_cfg c, cplx *zr, cplx *zt, cplx *zp, cplx *qlm, cplx *slm, cplx *tlm);
 extern "C" void SH_to_spat_cplx(shtns_cfg c, cplx *alm, cplx *z) {
     clear_error();
     if (!cplx_ready(c, "SH_to_spat_cplx")) return;
@@ -1314,4 +1391,14 @@ extern "C" void spat_cplx_to_SH(shtns_cfg c, cplx *z, cplx *alm) {
     if (g_last_err) return;
     if (!d2h(alm, da, ab, c->comp_stream)) { set_error(2, "spat_cplx_to_SH: copy failed"); return; }
     HIP_OK(hipStreamSynchronize(c->comp_stream));
+}
+extern "C" void SHqst_to_spat_cplx(shtns_cfg c, cplx *qlm, cplx *slm, cplx *tlm, cplx *zr, cplx *zt, cplx *zp) {
+    SH_to_spat_cplx(c, qlm, zr);
+    if (g_last_err) return;
+    SHsphtor_to_spat_cplx(c, slm, tlm, zt, zp);
+}
+extern "C" void spat_cplx_to_SHqst(shtns_cfg c, cplx *zr, cplx *zt, cplx *zp, cplx *qlm, cplx *slm, cplx *tlm) {
+    spat_cplx_to_SH(c, zr, qlm);
+    if (g_last_err) return;
+    spat_cplx_to_SHsphtor(c, zt, zp, slm, tlm);
 }

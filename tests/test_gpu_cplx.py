@@ -1,5 +1,6 @@
-"""GPU: complex-valued scalar transforms SH_to_spat_cplx / spat_cplx_to_SH (shtns.h:272, 277;
-sht_func.c:943-1052), their device forms cu_SH_to_spat_cplx / cu_spat_cplx_to_SH and the
+"""GPU: complex-valued transforms SH_to_spat_cplx / spat_cplx_to_SH (shtns.h:272, 277;
+sht_func.c:943-1052), SHsphtor_to_spat_cplx / spat_cplx_to_SHsphtor (shtns.h:292-294;
+sht_func.c:1060-1195), the 3-component forms (shtns.h:308-309), their device forms and the
 Python synth_cplx / analys_cplx (shtns_numpy.i:469-535), against the oracle's complex
 transforms (pinned against scipy's complex spherical harmonics on the CPU)."""
 import ctypes
@@ -113,8 +114,80 @@ def test_cplx_rejects(device):
     s, _ = make(31, 31, 32, 64)
     a = np.zeros(s.nlm_cplx, dtype=np.complex128)
     with pytest.raises(RuntimeError):
-        s.synth_cplx(a, a)
+        s.synth_cplx(a, a, a, a)
     with pytest.raises(RuntimeError):
         s.synth_cplx(np.zeros(s.nlm, dtype=np.complex128))
     with pytest.raises(RuntimeError):
         s.analys_cplx(np.zeros(s.spat_shape))    # float64, not complex
+
+
+@pytest.mark.parametrize("case", [(31, 32, 64), (63, 96, 192)])
+def test_sphtor_cplx_matches_oracle(case, device):
+    L, nlat, nphi = case
+    s, o = make(L, L, nlat, nphi)
+    S, T = random_alm(s.nlm_cplx, 80, L, o), random_alm(s.nlm_cplx, 81, L, o)
+    S[0] = T[0] = 0
+    zt, zp = s.synth_cplx(S, T)
+    rt, rp = o.synth_sphtor_cplx(S, T)
+    assert rel_l2(zt, rt) <= TOL and rel_l2(zp, rp) <= TOL
+    rng = np.random.default_rng(82)
+    wt, wp = rng.standard_normal((2, nphi, nlat)) + 1j * rng.standard_normal((2, nphi, nlat))
+    s2, t2 = s.analys_cplx(wt, wp)
+    rs, rt2 = o.analys_sphtor_cplx(wt, wp)
+    assert rel_l2(s2, rs) <= TOL and rel_l2(t2, rt2) <= TOL
+    s3, t3 = s.analys_cplx(zt, zp)
+    assert rel_l2(s3, S) <= 1e-13 and rel_l2(t3, T) <= 1e-13
+
+
+def test_sphtor_cplx_mmax_below_lmax_c_api(device):
+    s, o = make(40, 20, 64, 48)
+    S, T = random_alm(s.nlm_cplx, 83, 40, o), random_alm(s.nlm_cplx, 84, 40, o)
+    S[0] = T[0] = 0
+    zt, zp = (np.empty((48, 64), dtype=np.complex128) for _ in range(2))
+    lib = _lib.load()
+    p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    lib.SHsphtor_to_spat_cplx(s._cfg, p(S), p(T), p(zt), p(zp))
+    assert lib.shtns_last_error() == 0
+    rt, rp = o.synth_sphtor_cplx(S, T)
+    assert rel_l2(zt, rt) <= TOL and rel_l2(zp, rp) <= TOL
+    S2, T2 = np.empty_like(S), np.empty_like(T)
+    lib.spat_cplx_to_SHsphtor(s._cfg, p(zt), p(zp), p(S2), p(T2))
+    assert rel_l2(S2, S) <= 1e-13 and rel_l2(T2, T) <= 1e-13
+
+
+def test_sphtor_cplx_real_field(device):
+    """A real vector field: m >= 0 coefficients as the real vector analysis, m < 0 ones
+    (-1)^m conj."""
+    s, o = make(63, 63, 96, 192)
+    rng = np.random.default_rng(85)
+    vt, vp = rng.standard_normal((2, 192, 96))
+    sr, tr = s.analys(vt, vp)
+    sc, tc = s.analys_cplx(vt + 0j, vp + 0j)
+    scale = np.linalg.norm(sr)
+    for l in range(64):
+        for m in range(l + 1):
+            for c, r in ((sc, sr), (tc, tr)):
+                assert abs(c[s.zidx(l, m)] - r[s.idx(l, m)]) <= 1e-14 * scale
+                assert abs(c[s.zidx(l, -m)] - (-1) ** m * np.conj(r[s.idx(l, m)])) <= 1e-14 * scale
+
+
+def test_qst_cplx_torch_equals_host_and_components(device):
+    import torch
+    s, o = make(63, 63, 64, 128)
+    Q, S, T = (random_alm(s.nlm_cplx, 86 + i, 63, o) for i in range(3))
+    S[0] = T[0] = 0
+    zr, zt, zp = s.synth_cplx(Q, S, T)
+    assert np.array_equal(zr, s.synth_cplx(Q))
+    ut, up = s.synth_cplx(S, T)
+    assert np.array_equal(zt, ut) and np.array_equal(zp, up)
+    assert rel_l2(zr, o.synth_cplx(Q)) <= TOL
+    d = [torch.from_numpy(x).to(device) for x in (Q, S, T)]
+    wr, wt, wp = s.synth_cplx(*d)
+    for a, b in ((wr, zr), (wt, zt), (wp, zp)):
+        assert np.array_equal(a.cpu().numpy(), b)
+    q2, s2, t2 = s.analys_cplx(wr, wt, wp)
+    h = s.analys_cplx(zr, zt, zp)
+    for a, b in zip((q2, s2, t2), h):
+        assert np.array_equal(a.cpu().numpy(), b)
+    for a, b in zip(h, (Q, S, T)):
+        assert rel_l2(a, b) <= 1e-13

@@ -55,3 +55,64 @@ def test_real_field_reduces_to_real_transform():
             assert abs(a[o.lm_cplx(l, -m)] - (-1) ** m * np.conj(q[o.idx(l, m)])) <= 1e-13
     z = o.synth_cplx(a)
     assert np.max(np.abs(z.imag)) <= 1e-13 and rel_l2(z.real, v) <= 1e-13
+
+
+def _dtheta_Y(l, m, th, ph):
+    """dY_l^m/dtheta from Y_l^{m+-1} (CS-phased orthonormal Y, scipy)."""
+    out = np.zeros(np.broadcast(th, ph).shape, dtype=complex)
+    if m + 1 <= l:
+        out += 0.5 * np.sqrt((l - m) * (l + m + 1)) * np.exp(-1j * ph) * sp.sph_harm_y(l, m + 1, th, ph)
+    if m - 1 >= -l:
+        out -= 0.5 * np.sqrt((l + m) * (l - m + 1)) * np.exp(1j * ph) * sp.sph_harm_y(l, m - 1, th, ph)
+    return out
+
+
+@pytest.mark.parametrize("case", [(12, 12, 16, 32), (14, 6, 16, 16)])
+def test_sphtor_cplx_matches_analytic(case):
+    """Vt = dS/dtheta + (1/sin) dT/dphi, Vp = (1/sin) dS/dphi - dT/dtheta over all -l <= m <= l."""
+    L, M, nlat, nphi = case
+    o = Oracle(L, M, 1, nlat, nphi)
+    S, T = random_alm(o, 5), random_alm(o, 6)
+    S[0] = T[0] = 0
+    TH, PH = np.meshgrid(np.arccos(o.cos_theta), 2 * np.pi * np.arange(nphi) / nphi)
+    zt = np.zeros(TH.shape, dtype=complex)
+    zp = np.zeros_like(zt)
+    for l in range(1, L + 1):
+        for m in range(-min(l, M), min(l, M) + 1):
+            Y, dY = sp.sph_harm_y(l, m, TH, PH), _dtheta_Y(l, m, TH, PH)
+            cs, ct = S[o.lm_cplx(l, m)], T[o.lm_cplx(l, m)]
+            zt += cs * dY + ct * 1j * m * Y / np.sin(TH)
+            zp += cs * 1j * m * Y / np.sin(TH) - ct * dY
+    a, b = o.synth_sphtor_cplx(S, T)
+    assert rel_l2(a, zt) <= 1e-13 and rel_l2(b, zp) <= 1e-13
+    s2, t2 = o.analys_sphtor_cplx(zt, zp)
+    assert rel_l2(s2, S) <= 1e-13 and rel_l2(t2, T) <= 1e-13
+
+
+def test_sphtor_cplx_real_field_reduces_to_real_transform():
+    """Random (not band-limited) real vector field: m >= 0 coefficients equal the real vector
+    analysis, m < 0 ones (-1)^m conj; synthesis of a real field's coefficients is real."""
+    o = Oracle(31, 31, 1, 32, 64)
+    rng = np.random.default_rng(9)
+    vt, vp = rng.standard_normal((2, 64, 32))
+    sr, tr = o.analys_sphtor(vt, vp)
+    sc, tc = o.analys_sphtor_cplx(vt + 0j, vp + 0j)
+    for l in range(32):
+        for m in range(l + 1):
+            for c, r in ((sc, sr), (tc, tr)):
+                assert abs(c[o.lm_cplx(l, m)] - r[o.idx(l, m)]) <= 1e-14 * (1 + abs(r[o.idx(l, m)]))
+                assert abs(c[o.lm_cplx(l, -m)] - (-1) ** m * np.conj(r[o.idx(l, m)])) <= 1e-14 * (1 + abs(r[o.idx(l, m)]))
+    s = spectral_field(31, 31, seed=10, power=3.0)
+    t = spectral_field(31, 31, seed=11, power=3.0)
+    s[0] = t[0] = 0
+    wt, wp = o.synth_sphtor(s, t)
+    S = np.zeros(o.nlm_cplx, dtype=complex)
+    T = np.zeros_like(S)
+    for l in range(32):
+        for m in range(l + 1):
+            for C, r in ((S, s), (T, t)):
+                C[o.lm_cplx(l, m)] = r[o.idx(l, m)]
+                C[o.lm_cplx(l, -m)] = (-1) ** m * np.conj(r[o.idx(l, m)])
+    a, b = o.synth_sphtor_cplx(S, T)
+    assert rel_l2(a, wt) <= 1e-13 and rel_l2(b, wp) <= 1e-13
+    assert np.max(np.abs(a.imag)) <= 1e-13 * np.max(np.abs(wt))
