@@ -151,8 +151,23 @@ void spat_to_SHsphtor_l(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm,
 void spat_to_SHqst_l(shtns_cfg, double *Vr, double *Vt, double *Vp, cplx *Qlm, cplx *Slm, cplx *Tlm, int ltr);
 void SHqst_to_spat_l(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double *Vr, double *Vt, double *Vp, int ltr);
 
-/* point evaluation (shtns.h:350) */
+/* point and latitude evaluation, on the host; no grid needed (shtns.h:352-362) */
 double SH_to_point(shtns_cfg, cplx *Qlm, double cost, double phi);
+#if defined(__cplusplus) && defined(__clang__)
+/* std::complex<double> and C99 double complex share layout and return convention */
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wreturn-type-c-linkage"
+#endif
+cplx SH_to_point_cplx(shtns_cfg, cplx *alm, double cost, double phi);
+#if defined(__cplusplus) && defined(__clang__)
+#pragma clang diagnostic pop
+#endif
+void SH_to_grad_point(shtns_cfg, cplx *DrSlm, cplx *Slm, double cost, double phi, double *vr, double *vt, double *vp);
+void SHqst_to_point(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double cost, double phi, double *vr, double *vt,
+                    double *vp);
+void SH_to_lat(shtns_cfg shtns, cplx *Qlm, double cost, double *vr, int nphi, int ltr, int mtr);
+void SHqst_to_lat(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double cost, double *vr, double *vt, double *vp, int nphi,
+                  int ltr, int mtr);
 
 /* Status of the last call made from this thread: 0 = success.  Additive: SHTns has no
    status getter (its GPU calls print and return void, sht_gpu.cu:566, 610-615). */

@@ -172,6 +172,51 @@ class sht:
         q = np.ascontiguousarray(qlm, dtype=np.complex128)
         return _lib.load().SH_to_point(self._cfg, _ptr(q), float(cost), float(phi))
 
+    def SH_to_point_cplx(self, alm, cost, phi):
+        """Complex-valued field at one point from its (lmax+1)^2 coefficients (shtns_numpy.i:571-580)."""
+        a = np.ascontiguousarray(alm, dtype=np.complex128)
+        if a.size != self.nlm_cplx:
+            raise RuntimeError("spectral array has wrong size.")
+        r = _lib.load().SH_to_point_cplx(self._cfg, _ptr(a), float(cost), float(phi))
+        return complex(r.re, r.im)
+
+    def _point3(self, fn, arrays, cost, phi):
+        arr = []
+        for a in arrays:
+            a = np.ascontiguousarray(a, dtype=np.complex128)
+            if a.size != self.nlm:
+                raise RuntimeError("spectral array has wrong size.")
+            arr.append(a)
+        out = [ctypes.c_double() for _ in range(3)]
+        fn(self._cfg, *[_ptr(a) for a in arr], float(cost), float(phi), *[ctypes.byref(o) for o in out])
+        return tuple(o.value for o in out)
+
+    def SH_to_grad_point(self, DrSlm, Slm, cost, phi):
+        """(gr, gt, gp) = (sum DrSlm Y, dS/dtheta, (1/sin) dS/dphi) at one point (shtns_numpy.i:582-586)."""
+        return self._point3(_lib.load().SH_to_grad_point, (DrSlm, Slm), cost, phi)
+
+    def SHqst_to_point(self, Qlm, Slm, Tlm, cost, phi):
+        """(vr, vt, vp) of the 3-component field at one point (shtns_numpy.i:587-592)."""
+        return self._point3(_lib.load().SHqst_to_point, (Qlm, Slm, Tlm), cost, phi)
+
+    def SH_to_lat(self, Qlm, cost, Vr):
+        """Fill Vr (float64, nphi = its size) with the field on the latitude cos(theta) = cost at
+        equispaced longitudes 2 pi j / nphi (shtns_numpy.i:597-600)."""
+        q = np.ascontiguousarray(Qlm, dtype=np.complex128)
+        if q.size != self.nlm or Vr.dtype != np.float64 or not Vr.flags.c_contiguous:
+            raise RuntimeError("wrong array size or type.")
+        _lib.load().SH_to_lat(self._cfg, _ptr(q), float(cost), _ptr(Vr), int(Vr.size), self.lmax, self.mmax)
+
+    def SHqst_to_lat(self, Qlm, Slm, Tlm, cost, Vr, Vt, Vp):
+        """3-component field on one latitude (shtns_numpy.i:601-606)."""
+        q = [np.ascontiguousarray(a, dtype=np.complex128) for a in (Qlm, Slm, Tlm)]
+        n = Vr.size
+        if any(a.size != self.nlm for a in q) or any(v.size != n or v.dtype != np.float64 or not v.flags.c_contiguous
+                                                     for v in (Vr, Vt, Vp)):
+            raise RuntimeError("wrong array size or type.")
+        _lib.load().SHqst_to_lat(self._cfg, *[_ptr(a) for a in q], float(cost), _ptr(Vr), _ptr(Vt), _ptr(Vp), int(n),
+                                 self.lmax, self.mmax)
+
     # ---- checks ----
     def _need_grid(self):
         if self.nlat == 0:

@@ -20,6 +20,7 @@ EXPORTED_SYMBOLS = [
     "spat_to_SHqst", "SHqst_to_spat", "spat_to_SHqst_l", "SHqst_to_spat_l",
     "SH_to_spat_cplx", "spat_cplx_to_SH", "SHsphtor_to_spat_cplx", "spat_cplx_to_SHsphtor",
     "SHqst_to_spat_cplx", "spat_cplx_to_SHqst",
+    "SH_to_point_cplx", "SH_to_grad_point", "SHqst_to_point", "SH_to_lat", "SHqst_to_lat",
     "shtns_last_error", "shtns_last_error_string",
     # shtns_hip.h
     "cu_spat_to_SH", "cu_SH_to_spat", "cu_spat_to_SHsphtor", "cu_SHsphtor_to_spat",
@@ -33,6 +34,11 @@ EXPORTED_SYMBOLS = [
     "cushtns_set_precision", "cushtns_tune", "cushtns_tuning", "cushtns_set_tuning", "cushtns_batch_create", "cushtns_batch_arrays",
     "cushtns_batch_SH_to_spat", "cushtns_batch_spat_to_SH", "cushtns_batch_destroy",
 ]
+
+class Cplx(ctypes.Structure):
+    """C99 double complex (returned in two SSE registers, as a struct of two doubles)."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
 
 # public prefix of struct shtns_info (shtns.h:79-94)
 class ShtnsInfo(ctypes.Structure):
@@ -75,6 +81,7 @@ def load():
         pass
     lib = ctypes.CDLL(LIB_PATH)
     P, I, D, L = ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_long
+    Cplx_ = Cplx
     CFG = ctypes.POINTER(ShtnsInfo)
     sig = {
         "nlm_calc": (L, [L, L, L]),
@@ -100,6 +107,11 @@ def load():
         "shtns_gauss_wts": (I, [CFG, P]),
         "legendre_sphPlm_array": (I, [CFG, I, I, D, P]),
         "SH_to_point": (D, [CFG, P, D, D]),
+        "SH_to_point_cplx": (Cplx_, [CFG, P, D, D]),
+        "SH_to_grad_point": (None, [CFG, P, P, D, D, P, P, P]),
+        "SHqst_to_point": (None, [CFG, P, P, P, D, D, P, P, P]),
+        "SH_to_lat": (None, [CFG, P, D, P, I, I, I]),
+        "SHqst_to_lat": (None, [CFG, P, P, P, D, P, P, P, I, I, I]),
         "spat_to_SH": (None, [CFG, P, P]),
         "SH_to_spat": (None, [CFG, P, P]),
         "spat_to_SH_l": (None, [CFG, P, P, I]),

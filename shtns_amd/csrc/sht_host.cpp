@@ -534,6 +534,186 @@ extern "C" double SH_to_point(shtns_cfg c, cplx *Qlm, double cost, double phi) {
     return v;
 }
 
+// Point and latitude evaluation (sht_func.c:516-885), on the host.  For order m >= 1 the
+// recurrence runs on P_lm / sin(theta) (seed sin^(m-1) theta), so the 1/sin(theta) terms of
+// the vector fields stay finite at the poles, and
+//   dP_lm/dtheta = l cos(theta) (P_l/sin) - sqrt((2l+1)/(2l-1) (l^2 - m^2)) (P_{l-1}/sin);
+// for m = 0, dP_l0/dtheta = sqrt(l (l+1)) P_l1 (orthonormal functions, Condon-Shortley phase).
+static void plm_over_sin(int m, int ltop, long double x, long double s, long double *y) {
+    // y[l - m] = P_lm(x) / sin(theta) for m >= 1, P_l0(x) for m = 0
+    long double y0 = (long double)seed_amm(m) * (m >= 1 ? powl(s, m - 1) : 1.0L);
+    y[0] = y0;
+    if (ltop == m) return;
+    long double y1 = sqrtl(2.0L * m + 3) * x * y0;
+    y[1] = y1;
+    for (int l = m + 2; l <= ltop; ++l) {
+        const long double t1 = (long double)((long)(l + m) * (l - m));
+        const long double t2 = (long double)((long)(l - 1 + m) * (l - 1 - m));
+        const long double a = sqrtl(((long double)(2 * l + 1) * (2 * l - 1)) / t1);
+        const long double b = -sqrtl(((long double)(2 * l + 1) * t2) / ((long double)(2 * l - 3) * t1));
+        const long double y2 = a * x * y1 + b * y0;
+        y0 = y1;
+        y1 = y2;
+        y[l - m] = y1;
+    }
+}
+// ys[l-m] = P_lm / sin (m >= 1) or P_l0 (m = 0); dy[l-m] = dP_lm/dtheta; l = m..ltop
+static void plm_deriv(int m, int ltop, double cost, std::vector<long double> &ys, std::vector<long double> &dy) {
+    const long double x = cost, s = sqrtl((1.0L - x) * (1.0L + x));
+    ys.assign(ltop - m + 1, 0.0L);
+    dy.assign(ltop - m + 1, 0.0L);
+    if (ltop < m) return;
+    plm_over_sin(m, ltop, x, s, ys.data());
+    if (m >= 1) {
+        for (int l = m; l <= ltop; ++l) {
+            const long double c = sqrtl((long double)(2 * l + 1) / (2 * l - 1) * ((long double)l * l - (long double)m * m));
+            dy[l - m] = l * x * ys[l - m] - (l > m ? c * ys[l - m - 1] : 0.0L);
+        }
+    } else if (ltop >= 1) {
+        std::vector<long double> y1(ltop);
+        plm_over_sin(1, ltop, x, s, y1.data());
+        for (int l = 1; l <= ltop; ++l) dy[l] = sqrtl((long double)l * (l + 1)) * s * y1[l - 1];
+    }
+}
+
+// std::complex<double> and C99 double complex share layout and return convention
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Wreturn-type-c-linkage"
+extern "C" cplx SH_to_point_cplx(shtns_cfg c, cplx *alm, double cost, double phi) {
+    // z = sum_{l, -l<=m<=l} a_lm Y_l^m, Y_l^{-m} = (-1)^m conj(Y_l^m) (sht_func.c:516-580)
+    if (c->mres != 1) runerr("complex SH requires mres=1.");
+    const double *a = reinterpret_cast<const double *>(alm);
+    const int L = c->lmax, M = c->mmax;
+    auto lmc = [&](int l, int m) -> long { return (l <= M) ? (long)l * (l + 1) + m : (long)M * (2 * l - M) + l + m; };
+    std::vector<double> y(L + 1);
+    double zr = 0, zi = 0;
+    for (int m = 0; m <= M; ++m) {
+        legendre_sphPlm_array(c, L, m, cost, y.data());
+        double pr = 0, pi = 0, nr = 0, ni = 0;
+        for (int l = m; l <= L; ++l) {
+            pr += y[l - m] * a[2 * lmc(l, m)];
+            pi += y[l - m] * a[2 * lmc(l, m) + 1];
+            if (m > 0) {
+                nr += y[l - m] * a[2 * lmc(l, -m)];
+                ni += y[l - m] * a[2 * lmc(l, -m) + 1];
+            }
+        }
+        const double cm = cos(m * phi), sm = sin(m * phi), sg = (m & 1) ? -1.0 : 1.0;
+        zr += pr * cm - pi * sm;
+        zi += pr * sm + pi * cm;
+        if (m > 0) {   // e^{-i m phi} (-1)^m (nr + i ni)
+            zr += sg * (nr * cm + ni * sm);
+            zi += sg * (ni * cm - nr * sm);
+        }
+    }
+    cplx z;
+    reinterpret_cast<double *>(&z)[0] = zr;
+    reinterpret_cast<double *>(&z)[1] = zi;
+    return z;
+}
+#pragma clang diagnostic pop
+
+// order sums of the real fields at one latitude: for each m, Fq = sum Q P, Fs = sum S dP,
+// Ft = sum T dP, Gs = sum S (P/sin), Gt = sum T (P/sin) (m >= 1); any of Q, S, T may be null
+struct OrderSums {
+    double q[2], ds[2], dt[2], ps[2], pt[2];
+};
+static OrderSums order_sums(shtns_cfg c, int im, int ltr, const std::vector<long double> &ys,
+                            const std::vector<long double> &dy, double s, const cplx *Q, const cplx *S, const cplx *T) {
+    OrderSums r{};
+    const int m = im * c->mres;
+    const long base = LiM(c, 0, im);
+    const double *q = reinterpret_cast<const double *>(Q), *sl = reinterpret_cast<const double *>(S),
+                 *tl = reinterpret_cast<const double *>(T);
+    for (int l = m; l <= ltr; ++l) {
+        const long i = 2 * (base + l);
+        const double p = (double)(m >= 1 ? ys[l - m] * s : ys[l - m]), d = (double)dy[l - m],
+                     po = (double)(m >= 1 ? ys[l - m] : 0.0L);
+        for (int k = 0; k < 2; ++k) {
+            if (q) r.q[k] += p * q[i + k];
+            if (sl) { r.ds[k] += d * sl[i + k]; r.ps[k] += po * sl[i + k]; }
+            if (tl) { r.dt[k] += d * tl[i + k]; r.pt[k] += po * tl[i + k]; }
+        }
+    }
+    return r;
+}
+
+// Re(c_m F e^{i m phi}) with c_0 = 1, c_m = 2
+static inline double re_e(const double *F, int m, double cm, double sm) {
+    return (m == 0 ? 1.0 : 2.0) * (F[0] * cm - F[1] * sm);
+}
+
+// vr = sum Q Y;  vt = dS/dtheta + (1/sin) dT/dphi;  vp = (1/sin) dS/dphi - dT/dtheta
+// (sht_func.c:672-715); SH_to_grad_point is the T = 0 case with Q the radial derivative
+static void qst_point(shtns_cfg c, const cplx *Q, const cplx *S, const cplx *T, double cost, double phi, double *vr,
+                      double *vt, double *vp) {
+    const double s = sqrt((1. - cost) * (1. + cost));
+    std::vector<long double> ys, dy;
+    double r = 0, t = 0, p = 0;
+    for (int im = 0; im <= c->mmax; ++im) {
+        const int m = im * c->mres;
+        plm_deriv(m, c->lmax, cost, ys, dy);
+        const OrderSums o = order_sums(c, im, c->lmax, ys, dy, s, Q, S, T);
+        const double cm = cos(m * phi), sm = sin(m * phi);
+        // (i m) G: real part -m G.im, imaginary part m G.re
+        const double ims[2] = {-m * o.ps[1], m * o.ps[0]}, imt[2] = {-m * o.pt[1], m * o.pt[0]};
+        r += re_e(o.q, m, cm, sm);
+        t += re_e(o.ds, m, cm, sm) + re_e(imt, m, cm, sm);
+        p += re_e(ims, m, cm, sm) - re_e(o.dt, m, cm, sm);
+    }
+    *vr = r;
+    *vt = t;
+    *vp = p;
+}
+extern "C" void SHqst_to_point(shtns_cfg c, cplx *Qlm, cplx *Slm, cplx *Tlm, double cost, double phi, double *vr,
+                               double *vt, double *vp) {
+    qst_point(c, Qlm, Slm, Tlm, cost, phi, vr, vt, vp);
+}
+extern "C" void SH_to_grad_point(shtns_cfg c, cplx *DrSlm, cplx *Slm, double cost, double phi, double *gr, double *gt,
+                                 double *gp) {
+    qst_point(c, DrSlm, Slm, nullptr, cost, phi, gr, gt, gp);
+}
+
+// Synthesis on nphi equispaced longitudes phi_j = 2 pi j / nphi of one latitude, degrees
+// <= ltr and orders <= mtr (clamped as sht_func.c:740-743); the Fourier sums are evaluated
+// directly, with each angle reduced exactly (m j mod nphi).
+static void qst_lat(shtns_cfg c, const cplx *Q, const cplx *S, const cplx *T, double cost, double *vr, double *vt,
+                    double *vp, int nphi, int ltr, int mtr) {
+    if (nphi <= 0) return;
+    if (ltr > c->lmax) ltr = c->lmax;
+    if (mtr > c->mmax) mtr = c->mmax;
+    if (mtr * c->mres > ltr) mtr = ltr / c->mres;
+    if (mtr * 2 * c->mres >= nphi) mtr = (nphi - 1) / (2 * c->mres);
+    const double s = sqrt((1. - cost) * (1. + cost));
+    for (int j = 0; j < nphi; ++j) {
+        if (vr) vr[j] = 0;
+        if (vt) vt[j] = 0;
+        if (vp) vp[j] = 0;
+    }
+    if (ltr < 0 || mtr < 0) return;
+    std::vector<long double> ys, dy;
+    for (int im = 0; im <= mtr; ++im) {
+        const int m = im * c->mres;
+        plm_deriv(m, ltr, cost, ys, dy);
+        const OrderSums o = order_sums(c, im, ltr, ys, dy, s, Q, S, T);
+        const double ims[2] = {-m * o.ps[1], m * o.ps[0]}, imt[2] = {-m * o.pt[1], m * o.pt[0]};
+        for (int j = 0; j < nphi; ++j) {
+            const double ang = 2.0 * M_PI * (double)(((long)m * j) % nphi) / nphi;
+            const double cm = cos(ang), sm = sin(ang);
+            if (vr) vr[j] += re_e(o.q, m, cm, sm);
+            if (vt) vt[j] += re_e(o.ds, m, cm, sm) + re_e(imt, m, cm, sm);
+            if (vp) vp[j] += re_e(ims, m, cm, sm) - re_e(o.dt, m, cm, sm);
+        }
+    }
+}
+extern "C" void SH_to_lat(shtns_cfg c, cplx *Qlm, double cost, double *vr, int nphi, int ltr, int mtr) {
+    qst_lat(c, Qlm, nullptr, nullptr, cost, vr, nullptr, nullptr, nphi, ltr, mtr);
+}
+extern "C" void SHqst_to_lat(shtns_cfg c, cplx *Qlm, cplx *Slm, cplx *Tlm, double cost, double *vr, double *vt,
+                             double *vp, int nphi, int ltr, int mtr) {
+    qst_lat(c, Qlm, Slm, Tlm, cost, vr, vt, vp, nphi, ltr, mtr);
+}
+
 // ------------------------------------------------------------------------------------------
 // device initialization
 // ------------------------------------------------------------------------------------------
