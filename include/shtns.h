@@ -151,6 +151,9 @@ void spat_to_SHsphtor_l(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm,
 void spat_to_SHqst_l(shtns_cfg, double *Vr, double *Vt, double *Vp, cplx *Qlm, cplx *Slm, cplx *Tlm, int ltr);
 void SHqst_to_spat_l(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double *Vr, double *Vt, double *Vp, int ltr);
 
+/* rotation about the z axis by alpha, on the host (shtns.h:188-191); Rlm may be Qlm */
+void SH_Zrotate(shtns_cfg, cplx *Qlm, double alpha, cplx *Rlm);
+
 /* point and latitude evaluation, on the host; no grid needed (shtns.h:352-362) */
 double SH_to_point(shtns_cfg, cplx *Qlm, double cost, double phi);
 #if defined(__cplusplus) && defined(__clang__)

@@ -172,6 +172,15 @@ class sht:
         q = np.ascontiguousarray(qlm, dtype=np.complex128)
         return _lib.load().SH_to_point(self._cfg, _ptr(q), float(cost), float(phi))
 
+    def Zrotate(self, Qlm, alpha):
+        """Rotate the field about the z axis by alpha (shtns_numpy.i:609-614)."""
+        q = np.ascontiguousarray(Qlm, dtype=np.complex128)
+        if q.size != self.nlm:
+            raise RuntimeError("spectral array has wrong size.")
+        r = np.empty_like(q)
+        _lib.load().SH_Zrotate(self._cfg, _ptr(q), float(alpha), _ptr(r))
+        return r
+
     def SH_to_point_cplx(self, alm, cost, phi):
         """Complex-valued field at one point from its (lmax+1)^2 coefficients (shtns_numpy.i:571-580)."""
         a = np.ascontiguousarray(alm, dtype=np.complex128)

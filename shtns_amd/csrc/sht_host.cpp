@@ -714,6 +714,24 @@ extern "C" void SHqst_to_lat(shtns_cfg c, cplx *Qlm, cplx *Slm, cplx *Tlm, doubl
     qst_lat(c, Qlm, Slm, Tlm, cost, vr, vt, vp, nphi, ltr, mtr);
 }
 
+// Rotation about the z axis by alpha (the frame by -alpha): R_lm = Q_lm e^{-i m alpha}, so the
+// rotated field at longitude phi is the original at phi - alpha (sht_func.c:37-50).  Host
+// arrays; Rlm may be Qlm.
+extern "C" void SH_Zrotate(shtns_cfg c, cplx *Qlm, double alpha, cplx *Rlm) {
+    const double *q = reinterpret_cast<const double *>(Qlm);
+    double *r = reinterpret_cast<double *>(Rlm);
+    for (int im = 0; im <= c->mmax; ++im) {
+        const int m = im * c->mres;
+        const double ca = cos(m * alpha), sa = sin(m * alpha);
+        for (int l = m; l <= c->lmax; ++l) {
+            const long i = 2 * LiM(c, l, im);
+            const double re = q[i], ii = q[i + 1];
+            r[i] = re * ca + ii * sa;
+            r[i + 1] = ii * ca - re * sa;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // device initialization
 // ------------------------------------------------------------------------------------------

@@ -121,3 +121,20 @@ def test_lat_truncation_and_nphi_clamp():
     qc[s.m > 3] = 0
     ref = np.array([s.SH_to_point(qc, x, 2 * np.pi * j / 8) for j in range(8)])
     assert np.max(np.abs(vr8 - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("sizes", [(20, 20, 1), (24, 6, 4)])
+def test_zrotate(sizes):
+    """The rotated field at longitude phi is the original at phi - alpha; m = 0 is unchanged;
+    in place gives the same result."""
+    s = shtns_amd.sht(*sizes)
+    q = spectral_field(sizes[0], s.mmax, seed=12, mres=sizes[2])
+    alpha = 0.8
+    r = s.Zrotate(q, alpha)
+    assert np.array_equal(r[s.m == 0], q[s.m == 0])
+    for cost, phi in POINTS:
+        assert abs(s.SH_to_point(r, cost, phi) - s.SH_to_point(q, cost, phi - alpha)) <= 1e-13
+    from shtns_amd import _lib
+    q2 = q.copy()
+    _lib.load().SH_Zrotate(s._cfg, q2.ctypes.data, alpha, q2.ctypes.data)
+    assert np.array_equal(q2, r)
