@@ -118,6 +118,12 @@ int legendre_sphPlm_array(shtns_cfg shtns, const int lmax, const int im, const d
 void spat_to_SH(shtns_cfg shtns, double *Vr, cplx *Qlm);
 void SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr);
 
+/* Legendre transforms at order m = im*mres, no FFT (shtns.h:329-334): Vr has nlat complex
+   values (grid latitudes), Ql lmax+1-m (degrees m..lmax; spat_to_SH_ml writes 0 above ltr).
+   V_k = sum_l Q_l P_lm(x_k);  Q_l = 2 pi sum_k w_k P_lm(x_k) V_k.  At m = 0 only real parts. */
+void spat_to_SH_ml(shtns_cfg, int im, cplx *Vr, cplx *Ql, int ltr);
+void SH_to_spat_ml(shtns_cfg, int im, cplx *Ql, cplx *Vr, int ltr);
+
 /* complex-valued scalar fields, mres = 1 (shtns.h:272, 277).  alm[LM_cplx(l, m)] holds
    -l <= m <= l, nlm_cplx entries; z has the real transforms' layout with complex elements. */
 void SH_to_spat_cplx(shtns_cfg shtns, cplx *alm, cplx *z);

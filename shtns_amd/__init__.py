@@ -181,6 +181,27 @@ class sht:
         _lib.load().SH_Zrotate(self._cfg, _ptr(q), float(alpha), _ptr(r))
         return r
 
+    def spat_to_SH_m(self, Vr, Qlm, im):
+        """Legendre analysis at order m = im*mres, no FFT: Vr complex[nlat] -> Qlm complex[lmax+1-m]
+        (shtns_numpy.i:662-666)."""
+        self._need_grid()
+        m = int(im) * self.mres
+        if Vr.size != self.nlat or Qlm.size != self.lmax + 1 - m or Vr.dtype != np.complex128 \
+                or Qlm.dtype != np.complex128 or not Vr.flags.c_contiguous or not Qlm.flags.c_contiguous:
+            raise RuntimeError("wrong array size or type.")
+        _lib.load().spat_to_SH_ml(self._cfg, int(im), _ptr(Vr), _ptr(Qlm), self.lmax)
+        _lib.check_error("spat_to_SH_m")
+
+    def SH_to_spat_m(self, Qlm, Vr, im):
+        """Legendre synthesis at order m = im*mres, no FFT (shtns_numpy.i:667-671)."""
+        self._need_grid()
+        m = int(im) * self.mres
+        if Vr.size != self.nlat or Qlm.size != self.lmax + 1 - m or Vr.dtype != np.complex128 \
+                or Qlm.dtype != np.complex128 or not Vr.flags.c_contiguous or not Qlm.flags.c_contiguous:
+            raise RuntimeError("wrong array size or type.")
+        _lib.load().SH_to_spat_ml(self._cfg, int(im), _ptr(Qlm), _ptr(Vr), self.lmax)
+        _lib.check_error("SH_to_spat_m")
+
     def SH_to_point_cplx(self, alm, cost, phi):
         """Complex-valued field at one point from its (lmax+1)^2 coefficients (shtns_numpy.i:571-580)."""
         a = np.ascontiguousarray(alm, dtype=np.complex128)

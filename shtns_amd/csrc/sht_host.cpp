@@ -1564,6 +1564,100 @@ extern "C" void spat_cplx_to_SHsphtor(shtns_cfg c, cplx *zt, cplx *zp, cplx *slm
    (sht_func.c:1197-1210) */
 extern "C" void SHqst_to_spat_cplx(shtns_cfg c, cplx *qlm, cplx *slm, cplx *tlm, cplx *zr, cplx *zt, cplx *zp);
 extern "C" void spat_cplx_to_SHqst(shtns_cfg c, cplx *zr, cplx *zt, cplx *zp, cplx *qlm, cplx *slm, cplx *tlm);
+/* Legendre transforms at one order m = im*mres, no FFT (shtns.h:329-334; the reference's
+   own round trip time_SHT.c:929-930 and the complex transforms sht_func.c:943-1052 fix the
+   scales):  SH_to_spat_ml: V[k] = sum_{l=m}^{ltr} Q[l-m] P_lm(x_k) at every grid latitude;
+   spat_to_SH_ml: Q[l-m] = 2 pi sum_k w_k P_lm(x_k) V[k] for l <= ltr, 0 above.  At m = 0
+   only real parts are used and the imaginary parts written are 0.  Host arrays; the work
+   runs on the GPU. */
+struct MlStage {
+    HostStage st;
+    double *T = nullptr, *ct = nullptr, *sn = nullptr, *w = nullptr;
+    double2 *ab = nullptr, *Q = nullptr, *V = nullptr;
+    int m = 0, nl = 0, nq = 0;
+};
+static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g) {
+    if (!ready(c, who)) return false;
+    if (im < 0 || im > c->mmax) {
+        set_error(1, "%s: order index im=%d outside [0, %d]", who, im, c->mmax);
+        return false;
+    }
+    g.m = im * c->mres;
+    if (ltr > c->lmax) ltr = c->lmax;
+    g.nl = (ltr >= g.m) ? ltr - g.m + 1 : 0;
+    g.nq = c->lmax - g.m + 1;
+    const int nlat = c->nlat;
+    std::vector<double> w(nlat);
+    for (int k = 0; k < nlat; ++k) w[k] = c->wg[std::min(k, nlat - 1 - k)];
+    std::vector<double2> ab(std::max(g.nl, 2));
+    const int m = g.m;
+    ab[0] = make_double2(0.0, 0.0);
+    ab[1] = make_double2((double)sqrtl(2.0L * m + 3), 0.0);
+    for (int j = 2; j < g.nl; ++j) {
+        const int l = m + j;
+        const long double t1 = (long double)((long)(l + m) * (l - m));
+        const long double t2 = (long double)((long)(l - 1 + m) * (l - 1 - m));
+        ab[j] = make_double2((double)sqrtl(((long double)(2 * l + 1) * (2 * l - 1)) / t1),
+                             (double)-sqrtl(((long double)(2 * l + 1) * t2) / ((long double)(2 * l - 3) * t1)));
+    }
+    const size_t nt = (size_t)std::max(g.nl, 1) * nlat;
+    g.T = (double *)g.st.dev(nt * sizeof(double));
+    g.ct = (double *)g.st.dev(nlat * sizeof(double));
+    g.sn = (double *)g.st.dev(nlat * sizeof(double));
+    g.w = (double *)g.st.dev(nlat * sizeof(double));
+    g.ab = (double2 *)g.st.dev(ab.size() * sizeof(double2));
+    g.Q = (double2 *)g.st.dev((size_t)g.nq * sizeof(double2));
+    g.V = (double2 *)g.st.dev((size_t)nlat * sizeof(double2));
+    if (!g.T || !g.ct || !g.sn || !g.w || !g.ab || !g.Q || !g.V) {
+        set_error(2, "%s: device allocation failed", who);
+        return false;
+    }
+    const hipStream_t s = c->comp_stream;
+    if (!h2d(g.ct, c->ct, nlat * sizeof(double), s) || !h2d(g.sn, c->st, nlat * sizeof(double), s) ||
+        !h2d(g.w, w.data(), nlat * sizeof(double), s) || !h2d(g.ab, ab.data(), ab.size() * sizeof(double2), s)) {
+        set_error(2, "%s: copy failed", who);
+        return false;
+    }
+    // h2d copies from pageable host memory may return before the copy is done: wait before
+    // the host vectors go out of scope
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        set_error(2, "%s: copy failed", who);
+        return false;
+    }
+    const long double amm = seed_amm(m);
+    if (g.nl > 0) launch_ml_table(g.ct, g.sn, nlat, m, g.nl, g.ab, (double)log2l(fabsl(amm)), amm < 0 ? -1.0 : 1.0, g.T, s);
+    return true;
+}
+extern "C" void SH_to_spat_ml(shtns_cfg c, int im, cplx *Ql, cplx *Vr, int ltr) {
+    clear_error();
+    MlStage g{HostStage{c, {}}};
+    if (!ml_prepare(c, im, ltr, "SH_to_spat_ml", g)) return;
+    const hipStream_t s = c->comp_stream;
+    if (g.nl > 0) {
+        if (!h2d(g.Q, Ql, (size_t)g.nl * sizeof(cplx), s)) { set_error(2, "SH_to_spat_ml: copy failed"); return; }
+        launch_ml_synth(g.T, c->nlat, g.nl, g.Q, g.m == 0, g.V, s);
+    } else {
+        HIP_OK(hipMemsetAsync(g.V, 0, (size_t)c->nlat * sizeof(double2), s));
+    }
+    HIP_OK(hipGetLastError());
+    if (!d2h(Vr, g.V, (size_t)c->nlat * sizeof(cplx), s)) { set_error(2, "SH_to_spat_ml: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(s));
+}
+extern "C" void spat_to_SH_ml(shtns_cfg c, int im, cplx *Vr, cplx *Ql, int ltr) {
+    clear_error();
+    MlStage g{HostStage{c, {}}};
+    if (!ml_prepare(c, im, ltr, "spat_to_SH_ml", g)) return;
+    const hipStream_t s = c->comp_stream;
+    HIP_OK(hipMemsetAsync(g.Q, 0, (size_t)g.nq * sizeof(double2), s));
+    if (g.nl > 0) {
+        if (!h2d(g.V, Vr, (size_t)c->nlat * sizeof(cplx), s)) { set_error(2, "spat_to_SH_ml: copy failed"); return; }
+        launch_ml_anal(g.T, g.w, c->nlat, g.nl, g.V, g.m == 0, 2.0 * M_PI, g.Q, s);
+    }
+    HIP_OK(hipGetLastError());
+    if (!d2h(Ql, g.Q, (size_t)g.nq * sizeof(cplx), s)) { set_error(2, "spat_to_SH_ml: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(s));
+}
+
 extern "C" void SH_to_spat_cplx(shtns_cfg c, cplx *alm, cplx *z) {
     clear_error();
     if (!cplx_ready(c, "SH_to_spat_cplx")) return;

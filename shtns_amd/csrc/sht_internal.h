@@ -178,6 +178,13 @@ void launch_cplx_split(const double2 *a, double2 *X, double2 *Y, int lmax, int m
 void launch_cplx_merge(const double2 *X, const double2 *Y, double2 *a, int lmax, int mmax, hipStream_t s);
 void launch_interleave(const double *xy, double2 *z, long n, hipStream_t s);
 void launch_deinterleave(const double2 *z, double *xy, long n, hipStream_t s);
+// Legendre transforms at one order (no FFT): table of P_lm at every latitude, sum over degrees,
+// weighted sum over latitudes
+void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
+                     double seed_sign, double *T, hipStream_t s);
+void launch_ml_synth(const double *T, int nlat, int nl, const double2 *Q, int real_only, double2 *V, hipStream_t s);
+void launch_ml_anal(const double *T, const double *w, int nlat, int nl, const double2 *V, int real_only, double scale,
+                    double2 *Q, hipStream_t s);
 // the two halves of launch_legendre_synth: coefficient pre-mix, then the Legendre kernel for
 // latitude tiles [tile0, tile0 + ntl) (64 latitude pairs per tile)
 void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop, double *qmix,

@@ -294,6 +294,96 @@ void launch_deinterleave(const double2 *z, double *xy, long n, hipStream_t s) {
     hipLaunchKernelGGL(deinterleave_kernel, dim3(nb), dim3(256), 0, s, z, xy, n);
 }
 
+// ---- Legendre transforms at one order m (no FFT) ----
+// T[j][k] = P_{m+j,m}(x_k), j < nl, every grid latitude k: one thread per latitude runs the
+// three-term recurrence from the seed a_mm sin^m (a_mm = seed_sign 2^log2_seed, negative for
+// odd m with the Condon-Shortley phase), carrying a separate binary exponent so the
+// polar values underflow to zero only when they are below the double range.
+// ab[j] = (a_j, b_j): P_{m+j} = a_j x P_{m+j-1} + b_j P_{m+j-2} (ab[1].x = sqrt(2m+3)).
+__global__ void ml_table_kernel(const double *__restrict__ ct, const double *__restrict__ st, int nlat, int m, int nl,
+                                const double2 *__restrict__ ab, double log2_seed, double seed_sign,
+                                double *__restrict__ T) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlat) return;
+    const double x = ct[k], s = st[k];
+    if (m > 0 && !(s > 0.0)) {
+        for (int j = 0; j < nl; ++j) T[(size_t)j * nlat + k] = 0.0;
+        return;
+    }
+    const double lg = log2_seed + (m > 0 ? m * log2(s) : 0.0);
+    int e = (int)floor(lg);
+    double y0 = seed_sign * exp2(lg - e), y1 = 0.0;
+    T[k] = ldexp(y0, e);
+    if (nl > 1) {
+        y1 = ab[1].x * x * y0;
+        T[(size_t)nlat + k] = ldexp(y1, e);
+    }
+    for (int j = 2; j < nl; ++j) {
+        const double2 c = ab[j];
+        const double y2 = c.x * x * y1 + c.y * y0;
+        y0 = y1;
+        y1 = y2;
+        if (fabs(y1) > 0x1p256) {
+            y0 *= 0x1p-256;
+            y1 *= 0x1p-256;
+            e += 256;
+        }
+        T[(size_t)j * nlat + k] = ldexp(y1, e);
+    }
+}
+// V[k] = sum_j Q[j] T[j][k]; real_only (m = 0): real parts only
+__global__ void ml_synth_kernel(const double *__restrict__ T, int nlat, int nl, const double2 *__restrict__ Q,
+                                int real_only, double2 *__restrict__ V) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlat) return;
+    double re = 0.0, im = 0.0;
+    for (int j = 0; j < nl; ++j) {
+        const double t = T[(size_t)j * nlat + k];
+        const double2 q = Q[j];
+        re = fma(t, q.x, re);
+        im = fma(t, q.y, im);
+    }
+    V[k] = make_double2(re, real_only ? 0.0 : im);
+}
+// Q[j] = scale sum_k w_k T[j][k] V[k]: one block of 256 threads per degree, fixed-order tree
+__global__ __launch_bounds__(256) void ml_anal_kernel(const double *__restrict__ T, const double *__restrict__ w,
+                                                      int nlat, const double2 *__restrict__ V, int real_only,
+                                                      double scale, double2 *__restrict__ Q) {
+    __shared__ double sr[256], si[256];
+    const int j = blockIdx.x, t = threadIdx.x;
+    double re = 0.0, im = 0.0;
+    for (int k = t; k < nlat; k += 256) {
+        const double a = w[k] * T[(size_t)j * nlat + k];
+        const double2 v = V[k];
+        re = fma(a, v.x, re);
+        im = fma(a, v.y, im);
+    }
+    sr[t] = re;
+    si[t] = im;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) {
+            sr[t] += sr[t + h];
+            si[t] += si[t + h];
+        }
+        __syncthreads();
+    }
+    if (t == 0) Q[j] = make_double2(scale * sr[0], real_only ? 0.0 : scale * si[0]);
+}
+
+void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
+                     double seed_sign, double *T, hipStream_t s) {
+    hipLaunchKernelGGL(ml_table_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, ct, st, nlat, m, nl, ab, log2_seed,
+                       seed_sign, T);
+}
+void launch_ml_synth(const double *T, int nlat, int nl, const double2 *Q, int real_only, double2 *V, hipStream_t s) {
+    hipLaunchKernelGGL(ml_synth_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, T, nlat, nl, Q, real_only, V);
+}
+void launch_ml_anal(const double *T, const double *w, int nlat, int nl, const double2 *V, int real_only, double scale,
+                    double2 *Q, hipStream_t s) {
+    hipLaunchKernelGGL(ml_anal_kernel, dim3(nl), dim3(256), 0, s, T, w, nlat, V, real_only, scale, Q);
+}
+
 void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
                            double2 *F, long four_dist, double *qmix, int nsteps_total, hipStream_t s) {
     launch_premix(a, Qlm, spec_dist, spec_ltop, qmix, nsteps_total, s);

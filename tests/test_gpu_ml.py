@@ -1,0 +1,84 @@
+"""GPU: Legendre transforms at one order, no FFT: SH_to_spat_ml / spat_to_SH_ml (shtns.h:329-334,
+sht_com.c:96-102) and the Python SH_to_spat_m / spat_to_SH_m (shtns_numpy.i:662-671).
+The scales are those the reference's own uses fix: the round trip of time_SHT.c:929-930,
+and the complex transforms (sht_func.c:943-1052), where SH_to_spat_ml gives Fourier row m of
+a synthesis before the FFT and spat_to_SH_ml / nphi gives the analysis from the FFT's row m.
+Checked here against the oracle's full transforms of single-order fields."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import shtns_amd
+from shtns_amd import _lib
+from shtns_amd.synthetic import spectral_field
+from oracle.oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((np.asarray(a) - b).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
+
+
+def make(lmax, mmax, mres, nlat, nphi):
+    s = shtns_amd.sht(lmax, mmax, mres)
+    s.set_grid(nlat, nphi, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+    return s, Oracle(lmax, mmax, mres, nlat, nphi)
+
+
+@pytest.mark.parametrize("case", [(63, 63, 1, 96, 192), (60, 15, 4, 64, 48), (255, 255, 1, 384, 768)])
+@pytest.mark.parametrize("which", ["zero", "low", "high"])
+def test_ml_against_oracle(case, which, device):
+    L, M, mres, nlat, nphi = case
+    s, o = make(L, M, mres, nlat, nphi)
+    im = {"zero": 0, "low": 1, "high": M}[which]
+    m = im * mres
+    q = spectral_field(L, M, seed=90 + im, mres=mres)
+    sel = s.m == m
+    ql = np.ascontiguousarray(q[sel])
+    assert ql.size == L + 1 - m
+    # synthesis: Fourier row m of the oracle's synthesis of the single-order field
+    qm = np.zeros_like(q)
+    qm[sel] = q[sel]
+    v = o.synth(qm)                               # [nphi, nlat]
+    # phi spans 2 pi / mres, so FFT bin im holds order m; v = G_0 at m = 0 and
+    # v = 2 Re(G_m e^{i m phi}) = G_m e^{i m phi} + conj: bin im = nphi G_m
+    row = np.fft.fft(v, axis=0)[im] / nphi
+    G = np.empty(nlat, dtype=np.complex128)
+    s.SH_to_spat_m(ql, G, im)
+    ref = row.real + 0j if im == 0 else row
+    assert rel_l2(G, ref) <= TOL, rel_l2(G, ref)
+    # analysis: (1/nphi) spat_to_SH_ml of the FFT's row m gives the oracle's analysis at order m
+    rng = np.random.default_rng(91)
+    w = rng.standard_normal((nphi, nlat))
+    F = np.ascontiguousarray(np.fft.fft(w, axis=0)[im])
+    Q = np.empty(L + 1 - m, dtype=np.complex128)
+    s.spat_to_SH_m(F, Q, im)
+    assert rel_l2(Q / nphi, o.analys(w)[sel]) <= TOL
+    # round trip (time_SHT.c:929-930)
+    Q2 = np.empty_like(Q)
+    s.spat_to_SH_m(G, Q2, im)
+    assert rel_l2(Q2, ql) <= 1e-13
+
+
+def test_ml_truncation_and_errors(device):
+    s, o = make(63, 63, 1, 96, 192)
+    lib = _lib.load()
+    im, ltr = 5, 40
+    ql = spectral_field(63, 63, seed=92)[s.m == 5].copy()
+    G = np.empty(96, dtype=np.complex128)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    lib.SH_to_spat_ml(s._cfg, im, p(ql), p(G), ltr)
+    qt = ql.copy()
+    qt[ltr - 5 + 1:] = 0
+    G2 = np.empty_like(G)
+    s.SH_to_spat_m(qt, G2, im)
+    assert rel_l2(G, G2) <= 1e-14
+    Q = np.full(59, 7.0 + 7.0j)
+    lib.spat_to_SH_ml(s._cfg, im, p(G), p(Q), ltr)
+    assert np.all(Q[ltr - 5 + 1:] == 0)
+    assert rel_l2(Q[:ltr - 5 + 1], qt[:ltr - 5 + 1]) <= 1e-13
+    lib.SH_to_spat_ml(s._cfg, 64, p(ql), p(G), 63)     # im > mmax
+    assert lib.shtns_last_error() != 0
