@@ -123,6 +123,11 @@ void SH_to_spat(shtns_cfg shtns, cplx *Qlm, double *Vr);
    V_k = sum_l Q_l P_lm(x_k);  Q_l = 2 pi sum_k w_k P_lm(x_k) V_k.  At m = 0 only real parts. */
 void spat_to_SH_ml(shtns_cfg, int im, cplx *Vr, cplx *Ql, int ltr);
 void SH_to_spat_ml(shtns_cfg, int im, cplx *Ql, cplx *Vr, int ltr);
+/* vector forms (shtns.h:336-337): Vt_k = sum_l S_l dP/dtheta + T_l (i m/sin) P,
+   Vp_k = sum_l S_l (i m/sin) P - T_l dP/dtheta; the analysis is the matching Gauss projection
+   divided by l (l+1), times 2 pi */
+void spat_to_SHsphtor_ml(shtns_cfg, int im, cplx *Vt, cplx *Vp, cplx *Sl, cplx *Tl, int ltr);
+void SHsphtor_to_spat_ml(shtns_cfg, int im, cplx *Sl, cplx *Tl, cplx *Vt, cplx *Vp, int ltr);
 
 /* complex-valued scalar fields, mres = 1 (shtns.h:272, 277).  alm[LM_cplx(l, m)] holds
    -l <= m <= l, nlm_cplx entries; z has the real transforms' layout with complex elements. */

@@ -202,6 +202,26 @@ class sht:
         _lib.load().SH_to_spat_ml(self._cfg, int(im), _ptr(Qlm), _ptr(Vr), self.lmax)
         _lib.check_error("SH_to_spat_m")
 
+    def _check_m_arrays(self, im, spat, spec):
+        m = int(im) * self.mres
+        for a, n in [(v, self.nlat) for v in spat] + [(q, self.lmax + 1 - m) for q in spec]:
+            if a.size != n or a.dtype != np.complex128 or not a.flags.c_contiguous:
+                raise RuntimeError("wrong array size or type.")
+
+    def spat_to_SHsphtor_m(self, Vt, Vp, Slm, Tlm, im):
+        """Vector Legendre analysis at order m = im*mres, no FFT (shtns_numpy.i:672-676)."""
+        self._need_grid()
+        self._check_m_arrays(im, (Vt, Vp), (Slm, Tlm))
+        _lib.load().spat_to_SHsphtor_ml(self._cfg, int(im), _ptr(Vt), _ptr(Vp), _ptr(Slm), _ptr(Tlm), self.lmax)
+        _lib.check_error("spat_to_SHsphtor_m")
+
+    def SHsphtor_to_spat_m(self, Slm, Tlm, Vt, Vp, im):
+        """Vector Legendre synthesis at order m = im*mres, no FFT (shtns_numpy.i:677-681)."""
+        self._need_grid()
+        self._check_m_arrays(im, (Vt, Vp), (Slm, Tlm))
+        _lib.load().SHsphtor_to_spat_ml(self._cfg, int(im), _ptr(Slm), _ptr(Tlm), _ptr(Vt), _ptr(Vp), self.lmax)
+        _lib.check_error("SHsphtor_to_spat_m")
+
     def SH_to_point_cplx(self, alm, cost, phi):
         """Complex-valued field at one point from its (lmax+1)^2 coefficients (shtns_numpy.i:571-580)."""
         a = np.ascontiguousarray(alm, dtype=np.complex128)

@@ -20,6 +20,7 @@ EXPORTED_SYMBOLS = [
     "spat_to_SHqst", "SHqst_to_spat", "spat_to_SHqst_l", "SHqst_to_spat_l",
     "SH_to_spat_cplx", "spat_cplx_to_SH", "SHsphtor_to_spat_cplx", "spat_cplx_to_SHsphtor",
     "SHqst_to_spat_cplx", "spat_cplx_to_SHqst", "SH_to_spat_ml", "spat_to_SH_ml",
+    "SHsphtor_to_spat_ml", "spat_to_SHsphtor_ml",
     "SH_to_point_cplx", "SH_to_grad_point", "SHqst_to_point", "SH_to_lat", "SHqst_to_lat", "SH_Zrotate",
     "shtns_last_error", "shtns_last_error_string",
     # shtns_hip.h
@@ -114,6 +115,8 @@ def load():
         "SH_Zrotate": (None, [CFG, P, D, P]),
         "SH_to_spat_ml": (None, [CFG, I, P, P, I]),
         "spat_to_SH_ml": (None, [CFG, I, P, P, I]),
+        "SHsphtor_to_spat_ml": (None, [CFG, I, P, P, P, P, I]),
+        "spat_to_SHsphtor_ml": (None, [CFG, I, P, P, P, P, I]),
         "SHqst_to_lat": (None, [CFG, P, P, P, D, P, P, P, I, I, I]),
         "spat_to_SH": (None, [CFG, P, P]),
         "SH_to_spat": (None, [CFG, P, P]),

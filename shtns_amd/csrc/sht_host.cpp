@@ -1572,11 +1572,12 @@ extern "C" void spat_cplx_to_SHqst(shtns_cfg c, cplx *zr, cplx *zt, cplx *zp, cp
    runs on the GPU. */
 struct MlStage {
     HostStage st;
-    double *T = nullptr, *ct = nullptr, *sn = nullptr, *w = nullptr;
-    double2 *ab = nullptr, *Q = nullptr, *V = nullptr;
+    double *T = nullptr, *D = nullptr, *ct = nullptr, *sn = nullptr, *w = nullptr;
+    double2 *ab = nullptr, *Q = nullptr, *V = nullptr, *Q2 = nullptr, *V2 = nullptr;
     int m = 0, nl = 0, nq = 0;
 };
-static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g) {
+// vector: tables P/sin and dP/dtheta (the recurrence runs on order max(m, 1)), two fields
+static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g, bool vector = false) {
     if (!ready(c, who)) return false;
     if (im < 0 || im > c->mmax) {
         set_error(1, "%s: order index im=%d outside [0, %d]", who, im, c->mmax);
@@ -1590,7 +1591,7 @@ static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g
     std::vector<double> w(nlat);
     for (int k = 0; k < nlat; ++k) w[k] = c->wg[std::min(k, nlat - 1 - k)];
     std::vector<double2> ab(std::max(g.nl, 2));
-    const int m = g.m;
+    const int m = (vector && g.m == 0) ? 1 : g.m;     // order of the recurrence
     ab[0] = make_double2(0.0, 0.0);
     ab[1] = make_double2((double)sqrtl(2.0L * m + 3), 0.0);
     for (int j = 2; j < g.nl; ++j) {
@@ -1608,7 +1609,12 @@ static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g
     g.ab = (double2 *)g.st.dev(ab.size() * sizeof(double2));
     g.Q = (double2 *)g.st.dev((size_t)g.nq * sizeof(double2));
     g.V = (double2 *)g.st.dev((size_t)nlat * sizeof(double2));
-    if (!g.T || !g.ct || !g.sn || !g.w || !g.ab || !g.Q || !g.V) {
+    if (vector) {
+        g.D = (double *)g.st.dev(nt * sizeof(double));
+        g.Q2 = (double2 *)g.st.dev((size_t)g.nq * sizeof(double2));
+        g.V2 = (double2 *)g.st.dev((size_t)nlat * sizeof(double2));
+    }
+    if (!g.T || !g.ct || !g.sn || !g.w || !g.ab || !g.Q || !g.V || (vector && (!g.D || !g.Q2 || !g.V2))) {
         set_error(2, "%s: device allocation failed", who);
         return false;
     }
@@ -1625,7 +1631,11 @@ static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g
         return false;
     }
     const long double amm = seed_amm(m);
-    if (g.nl > 0) launch_ml_table(g.ct, g.sn, nlat, m, g.nl, g.ab, (double)log2l(fabsl(amm)), amm < 0 ? -1.0 : 1.0, g.T, s);
+    const double lseed = (double)log2l(fabsl(amm)), sgn = amm < 0 ? -1.0 : 1.0;
+    if (g.nl > 0) {
+        if (vector) launch_ml_vtable(g.ct, g.sn, nlat, g.m, g.nl, g.ab, lseed, sgn, g.T, g.D, s);
+        else launch_ml_table(g.ct, g.sn, nlat, m, g.nl, g.ab, lseed, sgn, g.T, s);
+    }
     return true;
 }
 extern "C" void SH_to_spat_ml(shtns_cfg c, int im, cplx *Ql, cplx *Vr, int ltr) {
@@ -1655,6 +1665,48 @@ extern "C" void spat_to_SH_ml(shtns_cfg c, int im, cplx *Vr, cplx *Ql, int ltr) 
     }
     HIP_OK(hipGetLastError());
     if (!d2h(Ql, g.Q, (size_t)g.nq * sizeof(cplx), s)) { set_error(2, "spat_to_SH_ml: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(s));
+}
+
+/* Vector Legendre transforms at one order m = im*mres (shtns.h:336-337):
+   Vt_k = sum_l S_l dP_lm/dtheta + T_l (i m / sin) P_lm,  Vp_k = sum_l S_l (i m / sin) P_lm - T_l dP_lm/dtheta;
+   S_l = 2 pi / (l (l+1)) sum_k w_k [Vt_k dP_lm/dtheta + Vp_k (-i m / sin) P_lm],
+   T_l = 2 pi / (l (l+1)) sum_k w_k [Vt_k (-i m / sin) P_lm - Vp_k dP_lm/dtheta]
+   (the per-order pieces of the vector transforms; real parts only at m = 0). */
+extern "C" void SHsphtor_to_spat_ml(shtns_cfg c, int im, cplx *Sl, cplx *Tl, cplx *Vt, cplx *Vp, int ltr) {
+    clear_error();
+    MlStage g{HostStage{c, {}}};
+    if (!ml_prepare(c, im, ltr, "SHsphtor_to_spat_ml", g, true)) return;
+    const hipStream_t s = c->comp_stream;
+    const size_t vb = (size_t)c->nlat * sizeof(double2);
+    if (g.nl > 0) {
+        if (!h2d(g.Q, Sl, (size_t)g.nl * sizeof(cplx), s) || !h2d(g.Q2, Tl, (size_t)g.nl * sizeof(cplx), s)) {
+            set_error(2, "SHsphtor_to_spat_ml: copy failed");
+            return;
+        }
+        launch_ml_vsynth(g.T, g.D, c->nlat, g.nl, g.m, g.Q, g.Q2, g.V, g.V2, s);
+    } else {
+        HIP_OK(hipMemsetAsync(g.V, 0, vb, s));
+        HIP_OK(hipMemsetAsync(g.V2, 0, vb, s));
+    }
+    HIP_OK(hipGetLastError());
+    if (!d2h(Vt, g.V, vb, s) || !d2h(Vp, g.V2, vb, s)) { set_error(2, "SHsphtor_to_spat_ml: copy failed"); return; }
+    HIP_OK(hipStreamSynchronize(s));
+}
+extern "C" void spat_to_SHsphtor_ml(shtns_cfg c, int im, cplx *Vt, cplx *Vp, cplx *Sl, cplx *Tl, int ltr) {
+    clear_error();
+    MlStage g{HostStage{c, {}}};
+    if (!ml_prepare(c, im, ltr, "spat_to_SHsphtor_ml", g, true)) return;
+    const hipStream_t s = c->comp_stream;
+    const size_t qb = (size_t)g.nq * sizeof(double2), vb = (size_t)c->nlat * sizeof(double2);
+    HIP_OK(hipMemsetAsync(g.Q, 0, qb, s));
+    HIP_OK(hipMemsetAsync(g.Q2, 0, qb, s));
+    if (g.nl > 0) {
+        if (!h2d(g.V, Vt, vb, s) || !h2d(g.V2, Vp, vb, s)) { set_error(2, "spat_to_SHsphtor_ml: copy failed"); return; }
+        launch_ml_vanal(g.T, g.D, g.w, c->nlat, g.nl, g.m, g.V, g.V2, g.Q, g.Q2, s);
+    }
+    HIP_OK(hipGetLastError());
+    if (!d2h(Sl, g.Q, qb, s) || !d2h(Tl, g.Q2, qb, s)) { set_error(2, "spat_to_SHsphtor_ml: copy failed"); return; }
     HIP_OK(hipStreamSynchronize(s));
 }
 

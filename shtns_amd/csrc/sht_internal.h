@@ -185,6 +185,12 @@ void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl
 void launch_ml_synth(const double *T, int nlat, int nl, const double2 *Q, int real_only, double2 *V, hipStream_t s);
 void launch_ml_anal(const double *T, const double *w, int nlat, int nl, const double2 *V, int real_only, double scale,
                     double2 *Q, hipStream_t s);
+void launch_ml_vtable(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
+                      double seed_sign, double *Y, double *D, hipStream_t s);
+void launch_ml_vsynth(const double *Y, const double *D, int nlat, int nl, int m, const double2 *S, const double2 *T,
+                      double2 *Vt, double2 *Vp, hipStream_t s);
+void launch_ml_vanal(const double *Y, const double *D, const double *w, int nlat, int nl, int m, const double2 *Vt,
+                     const double2 *Vp, double2 *S, double2 *T, hipStream_t s);
 // the two halves of launch_legendre_synth: coefficient pre-mix, then the Legendre kernel for
 // latitude tiles [tile0, tile0 + ntl) (64 latitude pairs per tile)
 void launch_premix(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop, double *qmix,

@@ -371,6 +371,122 @@ __global__ __launch_bounds__(256) void ml_anal_kernel(const double *__restrict__
     if (t == 0) Q[j] = make_double2(scale * sr[0], real_only ? 0.0 : scale * si[0]);
 }
 
+// Vector tables at one order: Y[j][k] = P_lm(x_k) / sin(theta_k) (m >= 1; unused at m = 0) and
+// D[j][k] = dP_lm/dtheta at l = m + j.  For m >= 1 the recurrence runs on P/sin (seed
+// a_mm sin^(m-1)) and dP_l/dtheta = l x (P_l/sin) - sqrt((2l+1)/(2l-1) (l^2-m^2)) (P_{l-1}/sin).
+// For m = 0, dP_l0/dtheta = sqrt(l (l+1)) sin (P_l1/sin) from the order-1 recurrence (ab1,
+// seed a_11).  Every grid latitude has sin(theta) > 0.
+__global__ void ml_vtable_kernel(const double *__restrict__ ct, const double *__restrict__ st, int nlat, int m, int nl,
+                                 const double2 *__restrict__ ab, double log2_seed, double seed_sign,
+                                 double *__restrict__ Y, double *__restrict__ D) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlat) return;
+    const double x = ct[k], s = st[k];
+    const int mr = (m >= 1) ? m : 1;     // order of the recurrence run here
+    const double lg = log2_seed + (mr - 1) * log2(s);
+    int e = (int)floor(lg);
+    double y0 = seed_sign * exp2(lg - e), y1 = 0.0;
+    // j counts degrees l = mr + j of the recurrence; output index o = l - m
+    auto emit = [&](int l, double yl, double ylm1) {
+        const double c = sqrt((2.0 * l + 1.0) / (2.0 * l - 1.0) * ((double)l * l - (double)mr * mr));
+        if (m >= 1) {
+            Y[(size_t)(l - m) * nlat + k] = ldexp(yl, e);
+            D[(size_t)(l - m) * nlat + k] = ldexp(l * x * yl - c * ylm1, e);
+        } else {
+            D[(size_t)l * nlat + k] = sqrt((double)l * (l + 1)) * s * ldexp(yl, e);
+        }
+    };
+    if (m == 0) {
+        Y[k] = 0.0;
+        D[k] = 0.0;
+    }
+    const int lend = m + nl - 1;            // last degree written
+    if (lend < mr) return;
+    emit(mr, y0, 0.0);
+    if (lend == mr) return;
+    y1 = ab[1].x * x * y0;
+    emit(mr + 1, y1, y0);
+    for (int l = mr + 2; l <= lend; ++l) {
+        const double2 cf = ab[l - mr];
+        const double y2 = cf.x * x * y1 + cf.y * y0;
+        y0 = y1;
+        y1 = y2;
+        if (fabs(y1) > 0x1p256) {
+            y0 *= 0x1p-256;
+            y1 *= 0x1p-256;
+            e += 256;
+        }
+        emit(l, y1, y0);
+    }
+}
+// Vt[k] = sum_j S_j D + T_j (i m) Y;  Vp[k] = sum_j S_j (i m) Y - T_j D  (real parts only at m = 0)
+__global__ void ml_vsynth_kernel(const double *__restrict__ Y, const double *__restrict__ D, int nlat, int nl, int m,
+                                 const double2 *__restrict__ S, const double2 *__restrict__ T, double2 *__restrict__ Vt,
+                                 double2 *__restrict__ Vp) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nlat) return;
+    double tr = 0, ti = 0, pr = 0, pi = 0;
+    for (int j = 0; j < nl; ++j) {
+        const double d = D[(size_t)j * nlat + k];
+        const double my = (m >= 1) ? m * Y[(size_t)j * nlat + k] : 0.0;
+        const double2 sj = S[j], tj = T[j];
+        tr += d * sj.x - my * tj.y;
+        ti += d * sj.y + my * tj.x;
+        pr += -my * sj.y - d * tj.x;
+        pi += my * sj.x - d * tj.y;
+    }
+    if (m == 0) ti = pi = 0.0;
+    Vt[k] = make_double2(tr, ti);
+    Vp[k] = make_double2(pr, pi);
+}
+// S_j = 2 pi / (l (l+1)) sum_k w_k [Vt D + Vp (-i m) Y];  T_j = 2 pi / (l (l+1)) sum_k w_k [Vt (-i m) Y - Vp D]
+__global__ __launch_bounds__(256) void ml_vanal_kernel(const double *__restrict__ Y, const double *__restrict__ D,
+                                                       const double *__restrict__ w, int nlat, int m,
+                                                       const double2 *__restrict__ Vt, const double2 *__restrict__ Vp,
+                                                       double2 *__restrict__ S, double2 *__restrict__ T) {
+    __shared__ double red[4][256];
+    const int j = blockIdx.x, t = threadIdx.x, l = m + j;
+    double a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    for (int k = t; k < nlat; k += 256) {
+        const double a = w[k] * D[(size_t)j * nlat + k];
+        const double b = (m >= 1) ? w[k] * m * Y[(size_t)j * nlat + k] : 0.0;
+        const double2 vt = Vt[k], vp = Vp[k];
+        a0 += a * vt.x + b * vp.y;      // S.re
+        a1 += a * vt.y - b * vp.x;      // S.im
+        a2 += b * vt.y - a * vp.x;      // T.re
+        a3 += -b * vt.x - a * vp.y;     // T.im
+    }
+    red[0][t] = a0;
+    red[1][t] = a1;
+    red[2][t] = a2;
+    red[3][t] = a3;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h)
+            for (int q = 0; q < 4; ++q) red[q][t] += red[q][t + h];
+        __syncthreads();
+    }
+    if (t == 0) {
+        const double sc = (l > 0) ? 2.0 * M_PI / ((double)l * (l + 1)) : 0.0;
+        const bool ro = (m == 0);
+        S[j] = make_double2(sc * red[0][0], ro ? 0.0 : sc * red[1][0]);
+        T[j] = make_double2(sc * red[2][0], ro ? 0.0 : sc * red[3][0]);
+    }
+}
+
+void launch_ml_vtable(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
+                      double seed_sign, double *Y, double *D, hipStream_t s) {
+    hipLaunchKernelGGL(ml_vtable_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, ct, st, nlat, m, nl, ab, log2_seed,
+                       seed_sign, Y, D);
+}
+void launch_ml_vsynth(const double *Y, const double *D, int nlat, int nl, int m, const double2 *S, const double2 *T,
+                      double2 *Vt, double2 *Vp, hipStream_t s) {
+    hipLaunchKernelGGL(ml_vsynth_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, Y, D, nlat, nl, m, S, T, Vt, Vp);
+}
+void launch_ml_vanal(const double *Y, const double *D, const double *w, int nlat, int nl, int m, const double2 *Vt,
+                     const double2 *Vp, double2 *S, double2 *T, hipStream_t s) {
+    hipLaunchKernelGGL(ml_vanal_kernel, dim3(nl), dim3(256), 0, s, Y, D, w, nlat, m, Vt, Vp, S, T);
+}
 void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
                      double seed_sign, double *T, hipStream_t s) {
     hipLaunchKernelGGL(ml_table_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, ct, st, nlat, m, nl, ab, log2_seed,

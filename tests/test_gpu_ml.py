@@ -82,3 +82,36 @@ def test_ml_truncation_and_errors(device):
     assert rel_l2(Q[:ltr - 5 + 1], qt[:ltr - 5 + 1]) <= 1e-13
     lib.SH_to_spat_ml(s._cfg, 64, p(ql), p(G), 63)     # im > mmax
     assert lib.shtns_last_error() != 0
+
+
+@pytest.mark.parametrize("case", [(63, 63, 1, 96, 192), (60, 15, 4, 64, 48), (255, 255, 1, 384, 768)])
+@pytest.mark.parametrize("which", ["zero", "low", "high"])
+def test_ml_vector_against_oracle(case, which, device):
+    L, M, mres, nlat, nphi = case
+    s, o = make(L, M, mres, nlat, nphi)
+    im = {"zero": 0, "low": 1, "high": M}[which]
+    m = im * mres
+    sel = s.m == m
+    sl = spectral_field(L, M, seed=93 + im, mres=mres, power=3.0)
+    tl = spectral_field(L, M, seed=94 + im, mres=mres, power=3.0)
+    sl[0] = tl[0] = 0
+    S, T = (np.ascontiguousarray(a[sel]) for a in (sl, tl))
+    sm, tm = np.zeros_like(sl), np.zeros_like(tl)
+    sm[sel], tm[sel] = sl[sel], tl[sel]
+    vt, vp = o.synth_sphtor(sm, tm)
+    rt, rp = (np.fft.fft(a, axis=0)[im] / nphi for a in (vt, vp))
+    Gt, Gp = (np.empty(nlat, dtype=np.complex128) for _ in range(2))
+    s.SHsphtor_to_spat_m(S, T, Gt, Gp, im)
+    if im == 0:
+        rt, rp = rt.real + 0j, rp.real + 0j
+    assert rel_l2(Gt, rt) <= TOL and rel_l2(Gp, rp) <= TOL, (rel_l2(Gt, rt), rel_l2(Gp, rp))
+    rng = np.random.default_rng(95)
+    a, b = rng.standard_normal((2, nphi, nlat))
+    Ft, Fp = (np.ascontiguousarray(np.fft.fft(x, axis=0)[im]) for x in (a, b))
+    S2, T2 = (np.empty(L + 1 - m, dtype=np.complex128) for _ in range(2))
+    s.spat_to_SHsphtor_m(Ft, Fp, S2, T2, im)
+    s_ref, t_ref = o.analys_sphtor(a, b)
+    assert rel_l2(S2 / nphi, s_ref[sel]) <= TOL and rel_l2(T2 / nphi, t_ref[sel]) <= TOL
+    S3, T3 = np.empty_like(S2), np.empty_like(T2)
+    s.spat_to_SHsphtor_m(Gt, Gp, S3, T3, im)
+    assert rel_l2(S3, S) <= 1e-13 and rel_l2(T3, T) <= 1e-13
