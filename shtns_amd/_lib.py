@@ -20,7 +20,8 @@ EXPORTED_SYMBOLS = [
     "spat_to_SHqst", "SHqst_to_spat", "spat_to_SHqst_l", "SHqst_to_spat_l",
     "SH_to_spat_cplx", "spat_cplx_to_SH", "SHsphtor_to_spat_cplx", "spat_cplx_to_SHsphtor",
     "SHqst_to_spat_cplx", "spat_cplx_to_SHqst", "SH_to_spat_ml", "spat_to_SH_ml",
-    "SHsphtor_to_spat_ml", "spat_to_SHsphtor_ml",
+    "SHsphtor_to_spat_ml", "spat_to_SHsphtor_ml", "SHsph_to_spat_ml", "SHtor_to_spat_ml",
+    "SHqst_to_spat_ml", "spat_to_SHqst_ml",
     "SH_to_point_cplx", "SH_to_grad_point", "SHqst_to_point", "SH_to_lat", "SHqst_to_lat", "SH_Zrotate",
     "shtns_last_error", "shtns_last_error_string",
     # shtns_hip.h
@@ -117,6 +118,10 @@ def load():
         "spat_to_SH_ml": (None, [CFG, I, P, P, I]),
         "SHsphtor_to_spat_ml": (None, [CFG, I, P, P, P, P, I]),
         "spat_to_SHsphtor_ml": (None, [CFG, I, P, P, P, P, I]),
+        "SHsph_to_spat_ml": (None, [CFG, I, P, P, P, I]),
+        "SHtor_to_spat_ml": (None, [CFG, I, P, P, P, I]),
+        "SHqst_to_spat_ml": (None, [CFG, I, P, P, P, P, P, P, I]),
+        "spat_to_SHqst_ml": (None, [CFG, I, P, P, P, P, P, P, I]),
         "SHqst_to_lat": (None, [CFG, P, P, P, D, P, P, P, I, I, I]),
         "spat_to_SH": (None, [CFG, P, P]),
         "SH_to_spat": (None, [CFG, P, P]),

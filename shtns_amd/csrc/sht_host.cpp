@@ -1710,6 +1710,29 @@ extern "C" void spat_to_SHsphtor_ml(shtns_cfg c, int im, cplx *Vt, cplx *Vp, cpl
     HIP_OK(hipStreamSynchronize(s));
 }
 
+// shorthands of the one-order transforms (shtns.h:338-342): a zero spectrum for the missing
+// component; the 3-component forms are the scalar then the vector transform
+extern "C" void SHsph_to_spat_ml(shtns_cfg c, int im, cplx *Sl, cplx *Vt, cplx *Vp, int ltr) {
+    std::vector<cplx> z((size_t)c->lmax + 1);
+    SHsphtor_to_spat_ml(c, im, Sl, z.data(), Vt, Vp, ltr);
+}
+extern "C" void SHtor_to_spat_ml(shtns_cfg c, int im, cplx *Tl, cplx *Vt, cplx *Vp, int ltr) {
+    std::vector<cplx> z((size_t)c->lmax + 1);
+    SHsphtor_to_spat_ml(c, im, z.data(), Tl, Vt, Vp, ltr);
+}
+extern "C" void SHqst_to_spat_ml(shtns_cfg c, int im, cplx *Ql, cplx *Sl, cplx *Tl, cplx *Vr, cplx *Vt, cplx *Vp,
+                                 int ltr) {
+    SH_to_spat_ml(c, im, Ql, Vr, ltr);
+    if (g_last_err) return;
+    SHsphtor_to_spat_ml(c, im, Sl, Tl, Vt, Vp, ltr);
+}
+extern "C" void spat_to_SHqst_ml(shtns_cfg c, int im, cplx *Vr, cplx *Vt, cplx *Vp, cplx *Ql, cplx *Sl, cplx *Tl,
+                                 int ltr) {
+    spat_to_SH_ml(c, im, Vr, Ql, ltr);
+    if (g_last_err) return;
+    spat_to_SHsphtor_ml(c, im, Vt, Vp, Sl, Tl, ltr);
+}
+
 extern "C" void SH_to_spat_cplx(shtns_cfg c, cplx *alm, cplx *z) {
     clear_error();
     if (!cplx_ready(c, "SH_to_spat_cplx")) return;

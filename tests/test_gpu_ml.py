@@ -115,3 +115,31 @@ def test_ml_vector_against_oracle(case, which, device):
     S3, T3 = np.empty_like(S2), np.empty_like(T2)
     s.spat_to_SHsphtor_m(Gt, Gp, S3, T3, im)
     assert rel_l2(S3, S) <= 1e-13 and rel_l2(T3, T) <= 1e-13
+
+
+def test_ml_shorthands_equal_components(device):
+    s, o = make(63, 63, 1, 96, 192)
+    lib = _lib.load()
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    im = 3
+    n = 64 - im
+    rng = np.random.default_rng(96)
+    Q, S, T = (rng.standard_normal(n) + 1j * rng.standard_normal(n) for _ in range(3))
+    z = np.zeros(n, dtype=np.complex128)
+    out = lambda: np.empty(96, dtype=np.complex128)
+    a, b, c, d = out(), out(), out(), out()
+    lib.SHsph_to_spat_ml(s._cfg, im, p(S), p(a), p(b), 63)
+    s.SHsphtor_to_spat_m(S, z, c, d, im)
+    assert np.array_equal(a, c) and np.array_equal(b, d)
+    lib.SHtor_to_spat_ml(s._cfg, im, p(T), p(a), p(b), 63)
+    s.SHsphtor_to_spat_m(z, T, c, d, im)
+    assert np.array_equal(a, c) and np.array_equal(b, d)
+    vr, vt, vp = out(), out(), out()
+    lib.SHqst_to_spat_ml(s._cfg, im, p(Q), p(S), p(T), p(vr), p(vt), p(vp), 63)
+    s.SH_to_spat_m(Q, a, im)
+    s.SHsphtor_to_spat_m(S, T, c, d, im)
+    assert np.array_equal(vr, a) and np.array_equal(vt, c) and np.array_equal(vp, d)
+    q2, s2, t2 = (np.empty(n, dtype=np.complex128) for _ in range(3))
+    lib.spat_to_SHqst_ml(s._cfg, im, p(vr), p(vt), p(vp), p(q2), p(s2), p(t2), 63)
+    assert lib.shtns_last_error() == 0
+    assert rel_l2(q2, Q) <= 1e-13 and rel_l2(s2, S) <= 1e-13 and rel_l2(t2, T) <= 1e-13
