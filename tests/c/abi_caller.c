@@ -23,6 +23,14 @@ int main(void) {
     printf("sh00=%.17g sh10=%.17g sh11=%.17g\n", sh00_1(c), sh10_ct(c), sh11_st(c));
     printf("off_nlat=%zu off_li=%zu off_ct=%zu off_nlm_cplx=%zu\n", offsetof(struct shtns_info, nlat),
            offsetof(struct shtns_info, li), offsetof(struct shtns_info, ct), offsetof(struct shtns_info, nlm_cplx));
+    {   /* double complex return across the C ABI: a_{1,-1} = 1 gives Y_1^{-1}(theta, phi) */
+        cplx *alm = (cplx *)shtns_malloc(sizeof(cplx) * c->nlm_cplx);
+        for (unsigned i = 0; i < c->nlm_cplx; ++i) alm[i] = 0;
+        alm[LM_cplx(c, 1, -1)] = 1.0;
+        cplx z = SH_to_point_cplx(c, alm, 0.3, 0.5);
+        printf("zre=%.17g zim=%.17g\n", creal(z), cimag(z));
+        shtns_free(alm);
+    }
     shtns_destroy(c);
     shtns_reset();
     return 0;
