@@ -40,6 +40,14 @@ void cu_SH_to_spat_cplx(shtns_cfg, cplx *alm, cplx *z);
 void cu_spat_cplx_to_SH(shtns_cfg, cplx *z, cplx *alm);
 void cu_SHsphtor_to_spat_cplx(shtns_cfg, cplx *Slm, cplx *Tlm, cplx *Vt, cplx *Vp);
 void cu_spat_cplx_to_SHsphtor(shtns_cfg, cplx *Vt, cplx *Vp, cplx *Slm, cplx *Tlm);
+/* batched forms: field f at alm + f*spec_dist (spec_dist >= nlm_cplx) and z + f*spat_dist
+ * (spat_dist >= nspat), in complex elements */
+void cu_SH_to_spat_cplx_batch(shtns_cfg, cplx *alm, cplx *z, int nfields, long spec_dist, long spat_dist);
+void cu_spat_cplx_to_SH_batch(shtns_cfg, cplx *z, cplx *alm, int nfields, long spat_dist, long spec_dist);
+void cu_SHsphtor_to_spat_cplx_batch(shtns_cfg, cplx *Slm, cplx *Tlm, cplx *Vt, cplx *Vp, int nfields, long spec_dist,
+                                    long spat_dist);
+void cu_spat_cplx_to_SHsphtor_batch(shtns_cfg, cplx *Vt, cplx *Vp, cplx *Slm, cplx *Tlm, int nfields, long spat_dist,
+                                    long spec_dist);
 
 /* ---- initialization (shtns_cuda.h:63-72, sht_gpu.cu:227-336) ---- */
 shtns_cfg cushtns_clone(shtns_cfg shtns, hipStream_t compute_stream, hipStream_t transfer_stream); /* :63 */

@@ -359,29 +359,36 @@ class sht:
         """z = synth_cplx(zlm) ; zt, zp = synth_cplx(slm, tlm) ; zr, zt, zp = synth_cplx(qlm, slm, tlm):
         complex-valued fields from their (lmax+1)^2 coefficients per array
         (shtns_numpy.i:469-500).  numpy arrays use the host API; torch complex128 tensors on
-        the GPU stay on the device."""
+        the GPU stay on the device and may carry leading batch dimensions (one batched call)."""
         n = len(arg)
         self._need_cplx(n)
-        for a in arg:
-            if len(a.shape) != 1 or a.shape[0] != self.nlm_cplx:
-                raise RuntimeError("spectral array has wrong size.")
         if _is_torch(arg[0]):
             import torch
             q = []
             for a in arg:
                 if not _is_torch(a) or not a.is_cuda or a.dtype != torch.complex128:
                     raise RuntimeError("spectral tensors should be torch.complex128 on the GPU")
+                if a.shape[-1] != self.nlm_cplx or a.shape != arg[0].shape:
+                    raise RuntimeError("spectral array has wrong size.")
                 q.append(a.contiguous())
-            out = [torch.empty(tuple(self.spat_shape), dtype=torch.complex128, device=q[0].device) for _ in q]
+            batch = tuple(q[0].shape[:-1])           # leading batch dimensions, one launch sequence
+            nf = int(np.prod(batch)) if batch else 1
+            out = [torch.empty(batch + tuple(self.spat_shape), dtype=torch.complex128, device=q[0].device) for _ in q]
+            if nf == 0:
+                return out[0] if n == 1 else tuple(out)
             self._stream(q[0])
             lib = _lib.load()
             if n != 2:
-                lib.cu_SH_to_spat_cplx(self._cfg, _ptr(q[0]), _ptr(out[0]))
+                lib.cu_SH_to_spat_cplx_batch(self._cfg, _ptr(q[0]), _ptr(out[0]), nf, self.nlm_cplx, self.nspat)
                 _lib.check_error("synth_cplx")
             if n >= 2:
-                lib.cu_SHsphtor_to_spat_cplx(self._cfg, _ptr(q[-2]), _ptr(q[-1]), _ptr(out[-2]), _ptr(out[-1]))
+                lib.cu_SHsphtor_to_spat_cplx_batch(self._cfg, _ptr(q[-2]), _ptr(q[-1]), _ptr(out[-2]), _ptr(out[-1]),
+                                                   nf, self.nlm_cplx, self.nspat)
                 _lib.check_error("synth_cplx")
             return out[0] if n == 1 else tuple(out)
+        for a in arg:
+            if len(a.shape) != 1 or a.shape[0] != self.nlm_cplx:
+                raise RuntimeError("spectral array has wrong size.")
         q = []
         for a in arg:
             if a.dtype != np.complex128:
@@ -399,26 +406,33 @@ class sht:
         (shtns_numpy.i:502-535)."""
         n = len(arg)
         self._need_cplx(n)
-        for z in arg:
-            if tuple(z.shape) != tuple(self.spat_shape):
-                raise RuntimeError("spatial array has wrong shape.")
         if _is_torch(arg[0]):
             import torch
             v = []
             for z in arg:
                 if not _is_torch(z) or not z.is_cuda or z.dtype != torch.complex128:
                     raise RuntimeError("spatial tensors should be torch.complex128 on the GPU")
+                if tuple(z.shape[-2:]) != tuple(self.spat_shape) or z.shape != arg[0].shape:
+                    raise RuntimeError("spatial array has wrong shape.")
                 v.append(z.contiguous())
-            out = [torch.empty((self.nlm_cplx,), dtype=torch.complex128, device=v[0].device) for _ in v]
+            batch = tuple(v[0].shape[:-2])
+            nf = int(np.prod(batch)) if batch else 1
+            out = [torch.empty(batch + (self.nlm_cplx,), dtype=torch.complex128, device=v[0].device) for _ in v]
+            if nf == 0:
+                return out[0] if n == 1 else tuple(out)
             self._stream(v[0])
             lib = _lib.load()
             if n != 2:
-                lib.cu_spat_cplx_to_SH(self._cfg, _ptr(v[0]), _ptr(out[0]))
+                lib.cu_spat_cplx_to_SH_batch(self._cfg, _ptr(v[0]), _ptr(out[0]), nf, self.nspat, self.nlm_cplx)
                 _lib.check_error("analys_cplx")
             if n >= 2:
-                lib.cu_spat_cplx_to_SHsphtor(self._cfg, _ptr(v[-2]), _ptr(v[-1]), _ptr(out[-2]), _ptr(out[-1]))
+                lib.cu_spat_cplx_to_SHsphtor_batch(self._cfg, _ptr(v[-2]), _ptr(v[-1]), _ptr(out[-2]), _ptr(out[-1]),
+                                                   nf, self.nspat, self.nlm_cplx)
                 _lib.check_error("analys_cplx")
             return out[0] if n == 1 else tuple(out)
+        for z in arg:
+            if tuple(z.shape) != tuple(self.spat_shape):
+                raise RuntimeError("spatial array has wrong shape.")
         v = []
         for z in arg:
             if z.dtype != np.complex128:

@@ -27,7 +27,8 @@ EXPORTED_SYMBOLS = [
     # shtns_hip.h
     "cu_spat_to_SH", "cu_SH_to_spat", "cu_spat_to_SHsphtor", "cu_SHsphtor_to_spat",
     "cu_spat_to_SHqst", "cu_SHqst_to_spat", "cu_SH_to_spat_cplx", "cu_spat_cplx_to_SH",
-    "cu_SHsphtor_to_spat_cplx", "cu_spat_cplx_to_SHsphtor",
+    "cu_SHsphtor_to_spat_cplx", "cu_spat_cplx_to_SHsphtor", "cu_SH_to_spat_cplx_batch", "cu_spat_cplx_to_SH_batch",
+    "cu_SHsphtor_to_spat_cplx_batch", "cu_spat_cplx_to_SHsphtor_batch",
     "cushtns_clone", "cushtns_init_gpu", "cushtns_set_streams", "cushtns_release_gpu",
     "cu_SH_to_spat_batch", "cu_spat_to_SH_batch", "cu_SHsphtor_to_spat_batch",
     "cu_spat_to_SHsphtor_batch", "cu_SH_to_spat_adjoint_batch", "cu_spat_to_SH_adjoint_batch",
@@ -149,6 +150,10 @@ def load():
         "spat_cplx_to_SHsphtor": (None, [CFG, P, P, P, P]),
         "cu_SHsphtor_to_spat_cplx": (None, [CFG, P, P, P, P]),
         "cu_spat_cplx_to_SHsphtor": (None, [CFG, P, P, P, P]),
+        "cu_SH_to_spat_cplx_batch": (None, [CFG, P, P, I, L, L]),
+        "cu_spat_cplx_to_SH_batch": (None, [CFG, P, P, I, L, L]),
+        "cu_SHsphtor_to_spat_cplx_batch": (None, [CFG, P, P, P, P, I, L, L]),
+        "cu_spat_cplx_to_SHsphtor_batch": (None, [CFG, P, P, P, P, I, L, L]),
         "shtns_last_error": (I, []),
         "shtns_last_error_string": (ctypes.c_char_p, []),
         "cu_SH_to_spat": (None, [CFG, P, P, I]),

@@ -1459,68 +1459,99 @@ static bool cplx_ready(shtns_cfg c, const char *who) {
     if (c->mres != 1) runerr("complex SH requires mres=1.");
     return true;
 }
-extern "C" void cu_SH_to_spat_cplx(shtns_cfg c, cplx *alm, cplx *z) {
+static bool cplx_batch_ok(shtns_cfg c, const char *who, int nf, long spec_dist, long spat_dist) {
+    if (!cplx_ready(c, who)) return false;
+    if (nf > 0 && (spec_dist < (long)c->nlm_cplx || spat_dist < (long)c->nspat)) {
+        set_error(1, "%s: spec_dist %ld < nlm_cplx %u or spat_dist %ld < nspat %u", who, spec_dist, c->nlm_cplx,
+                  spat_dist, c->nspat);
+        return false;
+    }
+    return nf > 0;
+}
+/* Batches of complex fields: field f at alm + f*spec_dist and z + f*spat_dist (complex
+   elements).  The fields' real and imaginary parts go through the real kernels as one batch
+   of 2 nf real fields. */
+extern "C" void cu_SH_to_spat_cplx_batch(shtns_cfg c, cplx *alm, cplx *z, int nf, long spec_dist, long spat_dist) {
     clear_error();
-    if (!cplx_ready(c, "cu_SH_to_spat_cplx")) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
+    if (!cplx_batch_ok(c, "cu_SH_to_spat_cplx", nf, spec_dist, spat_dist)) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
     char *base = (char *)xscratch(c, qb + vb);
     if (!base) return;
     double2 *XY = (double2 *)base;
     double *xy = (double *)(base + qb);
-    launch_cplx_split(reinterpret_cast<const double2 *>(alm), XY, XY + c->nlm, c->lmax, c->mmax, c->comp_stream);
-    synth_core(c, reinterpret_cast<const cplx *>(XY), xy, c->lmax, 2, c->nlm, c->nspat, 1.0, 0, prec_f64(c));
+    launch_cplx_split(reinterpret_cast<const double2 *>(alm), spec_dist, XY, c->nlm, c->lmax, c->mmax, nf,
+                      c->comp_stream);
+    synth_core(c, reinterpret_cast<const cplx *>(XY), xy, c->lmax, 2 * nf, c->nlm, c->nspat, 1.0, 0, prec_f64(c));
     if (g_last_err) return;
-    launch_interleave(xy, reinterpret_cast<double2 *>(z), c->nspat, c->comp_stream);
+    launch_interleave(xy, reinterpret_cast<double2 *>(z), spat_dist, c->nspat, nf, c->comp_stream);
     HIP_OK(hipGetLastError());
+}
+extern "C" void cu_spat_cplx_to_SH_batch(shtns_cfg c, cplx *z, cplx *alm, int nf, long spat_dist, long spec_dist) {
+    clear_error();
+    if (!cplx_batch_ok(c, "cu_spat_cplx_to_SH", nf, spec_dist, spat_dist)) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
+    char *base = (char *)xscratch(c, qb + vb);
+    if (!base) return;
+    double2 *XY = (double2 *)base;
+    double *xy = (double *)(base + qb);
+    launch_deinterleave(reinterpret_cast<const double2 *>(z), spat_dist, xy, c->nspat, nf, c->comp_stream);
+    anal_core(c, xy, reinterpret_cast<cplx *>(XY), c->lmax, 2 * nf, c->nspat, c->nlm, 1.0, 1, prec_f64(c));
+    if (g_last_err) return;
+    launch_cplx_merge(XY, c->nlm, reinterpret_cast<double2 *>(alm), spec_dist, c->lmax, c->mmax, nf, c->comp_stream);
+    HIP_OK(hipGetLastError());
+}
+extern "C" void cu_SH_to_spat_cplx(shtns_cfg c, cplx *alm, cplx *z) {
+    cu_SH_to_spat_cplx_batch(c, alm, z, 1, c->nlm_cplx, c->nspat);
 }
 extern "C" void cu_spat_cplx_to_SH(shtns_cfg c, cplx *z, cplx *alm) {
-    clear_error();
-    if (!cplx_ready(c, "cu_spat_cplx_to_SH")) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
-    char *base = (char *)xscratch(c, qb + vb);
-    if (!base) return;
-    double2 *XY = (double2 *)base;
-    double *xy = (double *)(base + qb);
-    launch_deinterleave(reinterpret_cast<const double2 *>(z), xy, c->nspat, c->comp_stream);
-    anal_core(c, xy, reinterpret_cast<cplx *>(XY), c->lmax, 2, c->nspat, c->nlm, 1.0, 1, prec_f64(c));
-    if (g_last_err) return;
-    launch_cplx_merge(XY, XY + c->nlm, reinterpret_cast<double2 *>(alm), c->lmax, c->mmax, c->comp_stream);
-    HIP_OK(hipGetLastError());
+    cu_spat_cplx_to_SH_batch(c, z, alm, 1, c->nspat, c->nlm_cplx);
 }
 /* Complex-valued 2D vector fields: S and T are each split into the spectra of the real and
-   imaginary parts' vector fields; both parts go through the vector kernels as one batch of
-   two (SHTns: orders +m and -m per order, sht_func.c:1060-1200). */
-extern "C" void cu_SHsphtor_to_spat_cplx(shtns_cfg c, cplx *slm, cplx *tlm, cplx *zt, cplx *zp) {
+   imaginary parts' vector fields; all parts go through the vector kernels as one batch of
+   2 nf (SHTns: orders +m and -m per order, sht_func.c:1060-1200). */
+extern "C" void cu_SHsphtor_to_spat_cplx_batch(shtns_cfg c, cplx *slm, cplx *tlm, cplx *zt, cplx *zp, int nf,
+                                               long spec_dist, long spat_dist) {
     clear_error();
-    if (!cplx_ready(c, "cu_SHsphtor_to_spat_cplx")) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
+    if (!cplx_batch_ok(c, "cu_SHsphtor_to_spat_cplx", nf, spec_dist, spat_dist)) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
     char *base = (char *)xscratch(c, 2 * qb + 2 * vb);
     if (!base) return;
     double2 *SS = (double2 *)base, *TT = (double2 *)(base + qb);
     double *VT = (double *)(base + 2 * qb), *VP = (double *)(base + 2 * qb + vb);
-    launch_cplx_split(reinterpret_cast<const double2 *>(slm), SS, SS + c->nlm, c->lmax, c->mmax, c->comp_stream);
-    launch_cplx_split(reinterpret_cast<const double2 *>(tlm), TT, TT + c->nlm, c->lmax, c->mmax, c->comp_stream);
-    vsynth_core(c, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), VT, VP, c->lmax, 2, c->nlm, c->nspat);
+    const hipStream_t s = c->comp_stream;
+    launch_cplx_split(reinterpret_cast<const double2 *>(slm), spec_dist, SS, c->nlm, c->lmax, c->mmax, nf, s);
+    launch_cplx_split(reinterpret_cast<const double2 *>(tlm), spec_dist, TT, c->nlm, c->lmax, c->mmax, nf, s);
+    vsynth_core(c, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), VT, VP, c->lmax, 2 * nf, c->nlm,
+                c->nspat);
     if (g_last_err) return;
-    launch_interleave(VT, reinterpret_cast<double2 *>(zt), c->nspat, c->comp_stream);
-    launch_interleave(VP, reinterpret_cast<double2 *>(zp), c->nspat, c->comp_stream);
+    launch_interleave(VT, reinterpret_cast<double2 *>(zt), spat_dist, c->nspat, nf, s);
+    launch_interleave(VP, reinterpret_cast<double2 *>(zp), spat_dist, c->nspat, nf, s);
     HIP_OK(hipGetLastError());
 }
-extern "C" void cu_spat_cplx_to_SHsphtor(shtns_cfg c, cplx *zt, cplx *zp, cplx *slm, cplx *tlm) {
+extern "C" void cu_spat_cplx_to_SHsphtor_batch(shtns_cfg c, cplx *zt, cplx *zp, cplx *slm, cplx *tlm, int nf,
+                                               long spat_dist, long spec_dist) {
     clear_error();
-    if (!cplx_ready(c, "cu_spat_cplx_to_SHsphtor")) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm, vb = 2 * sizeof(double) * c->nspat;
+    if (!cplx_batch_ok(c, "cu_spat_cplx_to_SHsphtor", nf, spec_dist, spat_dist)) return;
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
     char *base = (char *)xscratch(c, 2 * qb + 2 * vb);
     if (!base) return;
     double2 *SS = (double2 *)base, *TT = (double2 *)(base + qb);
     double *VT = (double *)(base + 2 * qb), *VP = (double *)(base + 2 * qb + vb);
-    launch_deinterleave(reinterpret_cast<const double2 *>(zt), VT, c->nspat, c->comp_stream);
-    launch_deinterleave(reinterpret_cast<const double2 *>(zp), VP, c->nspat, c->comp_stream);
-    vanal_core(c, VT, VP, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), c->lmax, 2, c->nspat, c->nlm);
+    const hipStream_t s = c->comp_stream;
+    launch_deinterleave(reinterpret_cast<const double2 *>(zt), spat_dist, VT, c->nspat, nf, s);
+    launch_deinterleave(reinterpret_cast<const double2 *>(zp), spat_dist, VP, c->nspat, nf, s);
+    vanal_core(c, VT, VP, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), c->lmax, 2 * nf, c->nspat,
+               c->nlm);
     if (g_last_err) return;
-    launch_cplx_merge(SS, SS + c->nlm, reinterpret_cast<double2 *>(slm), c->lmax, c->mmax, c->comp_stream);
-    launch_cplx_merge(TT, TT + c->nlm, reinterpret_cast<double2 *>(tlm), c->lmax, c->mmax, c->comp_stream);
+    launch_cplx_merge(SS, c->nlm, reinterpret_cast<double2 *>(slm), spec_dist, c->lmax, c->mmax, nf, s);
+    launch_cplx_merge(TT, c->nlm, reinterpret_cast<double2 *>(tlm), spec_dist, c->lmax, c->mmax, nf, s);
     HIP_OK(hipGetLastError());
+}
+extern "C" void cu_SHsphtor_to_spat_cplx(shtns_cfg c, cplx *slm, cplx *tlm, cplx *zt, cplx *zp) {
+    cu_SHsphtor_to_spat_cplx_batch(c, slm, tlm, zt, zp, 1, c->nlm_cplx, c->nspat);
+}
+extern "C" void cu_spat_cplx_to_SHsphtor(shtns_cfg c, cplx *zt, cplx *zp, cplx *slm, cplx *tlm) {
+    cu_spat_cplx_to_SHsphtor_batch(c, zt, zp, slm, tlm, 1, c->nspat, c->nlm_cplx);
 }
 extern "C" void SHsphtor_to_spat_cplx(shtns_cfg c, cplx *slm, cplx *tlm, cplx *zt, cplx *zp) {
     clear_error();

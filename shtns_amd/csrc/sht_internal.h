@@ -174,10 +174,11 @@ void launch_convert(const float *src, long src_dist, double *dst, long dst_dist,
 void launch_convert(const double *src, long src_dist, float *dst, long dst_dist, long n, int nfields, hipStream_t s);
 // complex-valued fields: spectra split into / merged from the real and imaginary parts'
 // real-field spectra, and the spatial interleave of the two parts
-void launch_cplx_split(const double2 *a, double2 *X, double2 *Y, int lmax, int mmax, hipStream_t s);
-void launch_cplx_merge(const double2 *X, const double2 *Y, double2 *a, int lmax, int mmax, hipStream_t s);
-void launch_interleave(const double *xy, double2 *z, long n, hipStream_t s);
-void launch_deinterleave(const double2 *z, double *xy, long n, hipStream_t s);
+// (nf fields; field f's real-part / imaginary-part real fields are entries 2f and 2f+1 of a batch)
+void launch_cplx_split(const double2 *a, long a_dist, double2 *XY, long nlm, int lmax, int mmax, int nf, hipStream_t s);
+void launch_cplx_merge(const double2 *XY, long nlm, double2 *a, long a_dist, int lmax, int mmax, int nf, hipStream_t s);
+void launch_interleave(const double *xy, double2 *z, long z_dist, long n, int nf, hipStream_t s);
+void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int nf, hipStream_t s);
 // Legendre transforms at one order (no FFT): table of P_lm at every latitude, sum over degrees,
 // weighted sum over latitudes
 void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,

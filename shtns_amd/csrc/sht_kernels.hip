@@ -233,18 +233,22 @@ __device__ __forceinline__ long lm_cplx(int l, int m, int mmax) {
     return (l <= mmax) ? (long)l * (l + 1) + m : (long)mmax * (2 * l - mmax) + l + m;
 }
 
-__global__ void cplx_split_kernel(const double2 *__restrict__ a, double2 *__restrict__ X, double2 *__restrict__ Y,
+// Batches: field f of the complex spectra at a + f * a_dist; its two real spectra X_f, Y_f at
+// XY + 2 f nlm and XY + (2 f + 1) nlm (one batch of 2 nf real fields, field distance nlm).
+__global__ void cplx_split_kernel(const double2 *__restrict__ a, long a_dist, double2 *__restrict__ XY, long nlm,
                                   int lmax, int mmax) {
     const int m = blockIdx.x;
+    const double2 *af = a + (size_t)blockIdx.y * a_dist;
+    double2 *X = XY + (size_t)blockIdx.y * 2 * nlm, *Y = X + nlm;
     const double sgn = (m & 1) ? -1.0 : 1.0;
     for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
-        const double2 ap = a[lm_cplx(l, m, mmax)];
+        const double2 ap = af[lm_cplx(l, m, mmax)];
         const long lm = lm_real(l, m, lmax);
         if (m == 0) {
             X[lm] = make_double2(ap.x, 0.0);
             Y[lm] = make_double2(ap.y, 0.0);
         } else {
-            const double2 an = a[lm_cplx(l, -m, mmax)];
+            const double2 an = af[lm_cplx(l, -m, mmax)];
             const double bx = sgn * an.x, by = sgn * an.y;
             X[lm] = make_double2(0.5 * (ap.x + bx), 0.5 * (ap.y - by));
             Y[lm] = make_double2(0.5 * (ap.y + by), 0.5 * (bx - ap.x));
@@ -252,46 +256,54 @@ __global__ void cplx_split_kernel(const double2 *__restrict__ a, double2 *__rest
     }
 }
 
-__global__ void cplx_merge_kernel(const double2 *__restrict__ X, const double2 *__restrict__ Y, double2 *__restrict__ a,
+__global__ void cplx_merge_kernel(const double2 *__restrict__ XY, long nlm, double2 *__restrict__ a, long a_dist,
                                   int lmax, int mmax) {
     const int m = blockIdx.x;
+    double2 *af = a + (size_t)blockIdx.y * a_dist;
+    const double2 *X = XY + (size_t)blockIdx.y * 2 * nlm, *Y = X + nlm;
     const double sgn = (m & 1) ? -1.0 : 1.0;
     for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
         const long lm = lm_real(l, m, lmax);
         const double2 x = X[lm], y = Y[lm];
-        a[lm_cplx(l, m, mmax)] = make_double2(x.x - y.y, x.y + y.x);
-        if (m > 0) a[lm_cplx(l, -m, mmax)] = make_double2(sgn * (x.x + y.y), sgn * (y.x - x.y));
+        af[lm_cplx(l, m, mmax)] = make_double2(x.x - y.y, x.y + y.x);
+        if (m > 0) af[lm_cplx(l, -m, mmax)] = make_double2(sgn * (x.x + y.y), sgn * (y.x - x.y));
     }
 }
 
-// z[i] = (x[i], y[i]) and back; x and y are one batch of two real fields, field distance n
-__global__ void interleave_kernel(const double *__restrict__ xy, double2 *__restrict__ z, long n) {
+// z_f[i] = (x_f[i], y_f[i]) and back; x_f = xy + 2 f n, y_f = x_f + n; z_f = z + f z_dist
+__global__ void interleave_kernel(const double *__restrict__ xy, double2 *__restrict__ z, long z_dist, long n) {
+    const double *x = xy + (size_t)blockIdx.y * 2 * n;
+    double2 *zf = z + (size_t)blockIdx.y * z_dist;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
-        z[i] = make_double2(xy[i], xy[n + i]);
+        zf[i] = make_double2(x[i], x[n + i]);
 }
-__global__ void deinterleave_kernel(const double2 *__restrict__ z, double *__restrict__ xy, long n) {
+__global__ void deinterleave_kernel(const double2 *__restrict__ z, long z_dist, double *__restrict__ xy, long n) {
+    double *x = xy + (size_t)blockIdx.y * 2 * n;
+    const double2 *zf = z + (size_t)blockIdx.y * z_dist;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 v = z[i];
-        xy[i] = v.x;
-        xy[n + i] = v.y;
+        const double2 v = zf[i];
+        x[i] = v.x;
+        x[n + i] = v.y;
     }
 }
 
-void launch_cplx_split(const double2 *a, double2 *X, double2 *Y, int lmax, int mmax, hipStream_t s) {
-    hipLaunchKernelGGL(cplx_split_kernel, dim3(mmax + 1), dim3(256), 0, s, a, X, Y, lmax, mmax);
+void launch_cplx_split(const double2 *a, long a_dist, double2 *XY, long nlm, int lmax, int mmax, int nf, hipStream_t s) {
+    if (nf <= 0) return;
+    hipLaunchKernelGGL(cplx_split_kernel, dim3(mmax + 1, nf), dim3(256), 0, s, a, a_dist, XY, nlm, lmax, mmax);
 }
-void launch_cplx_merge(const double2 *X, const double2 *Y, double2 *a, int lmax, int mmax, hipStream_t s) {
-    hipLaunchKernelGGL(cplx_merge_kernel, dim3(mmax + 1), dim3(256), 0, s, X, Y, a, lmax, mmax);
+void launch_cplx_merge(const double2 *XY, long nlm, double2 *a, long a_dist, int lmax, int mmax, int nf, hipStream_t s) {
+    if (nf <= 0) return;
+    hipLaunchKernelGGL(cplx_merge_kernel, dim3(mmax + 1, nf), dim3(256), 0, s, XY, nlm, a, a_dist, lmax, mmax);
 }
-void launch_interleave(const double *xy, double2 *z, long n, hipStream_t s) {
-    if (n <= 0) return;
+void launch_interleave(const double *xy, double2 *z, long z_dist, long n, int nf, hipStream_t s) {
+    if (n <= 0 || nf <= 0) return;
     const int nb = (int)std::min<long>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(interleave_kernel, dim3(nb), dim3(256), 0, s, xy, z, n);
+    hipLaunchKernelGGL(interleave_kernel, dim3(nb, nf), dim3(256), 0, s, xy, z, z_dist, n);
 }
-void launch_deinterleave(const double2 *z, double *xy, long n, hipStream_t s) {
-    if (n <= 0) return;
+void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int nf, hipStream_t s) {
+    if (n <= 0 || nf <= 0) return;
     const int nb = (int)std::min<long>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(deinterleave_kernel, dim3(nb), dim3(256), 0, s, z, xy, n);
+    hipLaunchKernelGGL(deinterleave_kernel, dim3(nb, nf), dim3(256), 0, s, z, z_dist, xy, n);
 }
 
 // ---- Legendre transforms at one order m (no FFT) ----

@@ -191,3 +191,47 @@ def test_qst_cplx_torch_equals_host_and_components(device):
         assert np.array_equal(a.cpu().numpy(), b)
     for a, b in zip(h, (Q, S, T)):
         assert rel_l2(a, b) <= 1e-13
+
+
+def test_cplx_batch_equals_single(device):
+    """Batched device calls (leading dimensions of torch tensors, cu_*_cplx_batch) give each
+    field's single-call result bit for bit; padded distances through the C ABI."""
+    import torch
+    s, o = make(47, 47, 48, 96)
+    B = 5
+    A = np.stack([random_alm(s.nlm_cplx, 100 + f, 47, o) for f in range(B)]).reshape(B, s.nlm_cplx)
+    S = np.stack([random_alm(s.nlm_cplx, 110 + f, 47, o) for f in range(B)])
+    T = np.stack([random_alm(s.nlm_cplx, 120 + f, 47, o) for f in range(B)])
+    S[:, 0] = T[:, 0] = 0
+    dA, dS, dT = (torch.from_numpy(x).to(device) for x in (A, S, T))
+    z = s.synth_cplx(dA.reshape(B, 1, -1))
+    assert tuple(z.shape) == (B, 1, 96, 48)
+    zt, zp = s.synth_cplx(dS, dT)
+    for f in range(B):
+        assert torch.equal(z[f, 0], s.synth_cplx(dA[f]))
+        a1, b1 = s.synth_cplx(dS[f], dT[f])
+        assert torch.equal(zt[f], a1) and torch.equal(zp[f], b1)
+    a2 = s.analys_cplx(z)
+    s2, t2 = s.analys_cplx(zt, zp)
+    for f in range(B):
+        assert torch.equal(a2[f, 0], s.analys_cplx(z[f, 0]))
+        c1, d1 = s.analys_cplx(zt[f], zp[f])
+        assert torch.equal(s2[f], c1) and torch.equal(t2[f], d1)
+        assert rel_l2(a2[f, 0].cpu().numpy(), A[f]) <= 1e-13
+    assert rel_l2(zt[2].cpu().numpy(), o.synth_sphtor_cplx(S[2], T[2])[0]) <= TOL
+    # padded field distances through the C ABI
+    lib = _lib.load()
+    sd, vd = s.nlm_cplx + 7, s.nspat + 5
+    ap = torch.zeros((B, sd), dtype=torch.complex128, device=device)
+    ap[:, :s.nlm_cplx] = dA
+    zpd = torch.zeros((B, vd), dtype=torch.complex128, device=device)
+    st = torch.cuda.current_stream()
+    lib.cushtns_set_streams(s._cfg, ctypes.c_void_p(st.cuda_stream), ctypes.c_void_p(st.cuda_stream))
+    lib.cu_SH_to_spat_cplx_batch(s._cfg, ctypes.c_void_p(ap.data_ptr()), ctypes.c_void_p(zpd.data_ptr()), B, sd, vd)
+    torch.cuda.synchronize()
+    assert lib.shtns_last_error() == 0
+    assert torch.equal(zpd[:, :s.nspat].reshape(B, 96, 48), z[:, 0])
+    assert torch.all(zpd[:, s.nspat:] == 0)
+    lib.cu_SH_to_spat_cplx_batch(s._cfg, ctypes.c_void_p(ap.data_ptr()), ctypes.c_void_p(zpd.data_ptr()), B,
+                                 s.nlm_cplx - 1, vd)
+    assert lib.shtns_last_error() != 0
