@@ -78,6 +78,10 @@ struct shtns_info {
 /* index of (l, m), -l <= m <= l, in a complex-valued field's spectrum, mres = 1 (shtns.h:114) */
 #define LM_cplx(shtns, l, m) \
     (((l) <= shtns->mmax) ? (l) * ((l) + 1) + (m) : shtns->mmax * (2 * (l) - shtns->mmax) + (l) + (m))
+/* loops over every (l, m) of a spectrum: lm (declared by the macro) is the index; LM_L_LOOP
+   also declares the degree l (shtns.h:107-112) */
+#define LM_LOOP(shtns, action) { for (int lm = 0; lm < (int)shtns->nlm; ++lm) { action } }
+#define LM_L_LOOP(shtns, action) { for (int lm = 0; lm < (int)shtns->nlm; ++lm) { int l = shtns->li[lm]; action } }
 #define NSPAT_ALLOC(shtns) (shtns->nspat)
 #define PHI_DEG(shtns, ip) (360. / (shtns->nphi * shtns->mres)) * (ip)
 #define PHI_RAD(shtns, ip) (2. * M_PI / (shtns->nphi * shtns->mres)) * (ip)
@@ -150,6 +154,11 @@ void spat_to_SHsphtor(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm);
 void SHsphtor_to_spat(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, double *Vp);
 void SHsph_to_spat(shtns_cfg, cplx *Slm, double *Vt, double *Vp);
 void SHtor_to_spat(shtns_cfg, cplx *Tlm, double *Vt, double *Vp);
+
+/* gradient of a scalar field = the spheroidal transform (shtns.h:297, 327, 345) */
+#define SH_to_grad_spat(shtns, S, Gt, Gp) SHsph_to_spat(shtns, S, Gt, Gp)
+#define SH_to_grad_spat_l(shtns, S, Gt, Gp, ltr) SHsph_to_spat_l(shtns, S, Gt, Gp, ltr)
+#define SH_to_grad_spat_ml(shtns, im, S, Gt, Gp, ltr) SHsph_to_spat_ml(shtns, im, S, Gt, Gp, ltr)
 
 /* 3-component transforms: radial scalar Q and tangential spheroidal/toroidal S, T
    (shtns.h:303, 306) */

@@ -23,6 +23,18 @@ int main(void) {
     printf("sh00=%.17g sh10=%.17g sh11=%.17g\n", sh00_1(c), sh10_ct(c), sh11_st(c));
     printf("off_nlat=%zu off_li=%zu off_ct=%zu off_nlm_cplx=%zu\n", offsetof(struct shtns_info, nlat),
            offsetof(struct shtns_info, li), offsetof(struct shtns_info, ct), offsetof(struct shtns_info, nlm_cplx));
+    {   /* loop macros; the gradient aliases expand to declared functions (compiled, not run:
+           they need a GPU) */
+        long n = 0, lsum = 0;
+        LM_LOOP(c, n += 1 + 0 * lm;)
+        LM_L_LOOP(c, lsum += l + 0 * lm;)
+        printf("loopn=%ld lsum=%ld\n", n, lsum);
+        if (n < 0) {
+            SH_to_grad_spat(c, NULL, NULL, NULL);
+            SH_to_grad_spat_l(c, NULL, NULL, NULL, 3);
+            SH_to_grad_spat_ml(c, 1, NULL, NULL, NULL, 3);
+        }
+    }
     {   /* double complex return across the C ABI: a_{1,-1} = 1 gives Y_1^{-1}(theta, phi) */
         cplx *alm = (cplx *)shtns_malloc(sizeof(cplx) * c->nlm_cplx);
         for (unsigned i = 0; i < c->nlm_cplx; ++i) alm[i] = 0;

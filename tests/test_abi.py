@@ -58,6 +58,7 @@ def test_c_caller_against_public_header(tmp_path):
     assert abs(float(kv["y2"]) - o.legendre(2, 0.3)[0]) < 1e-15
     assert abs(float(kv["sh00"]) - np.sqrt(4 * np.pi)) < 1e-15
     assert abs(float(kv["sh11"]) + np.sqrt(2 * np.pi / 3)) < 1e-15
+    assert int(kv["loopn"]) == 528 and int(kv["lsum"]) == int(sum(l * min(l + 1, 32) for l in range(32)))
     # Y_1^{-1}(theta, phi) = sqrt(3 / (8 pi)) sin(theta) e^{-i phi} (Condon-Shortley phase)
     zref = np.sqrt(3 / (8 * np.pi)) * np.sqrt(1 - 0.3 ** 2) * np.exp(-0.5j)
     assert abs(complex(float(kv["zre"]), float(kv["zim"])) - zref) < 1e-15
