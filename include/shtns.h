@@ -116,6 +116,14 @@ int shtns_gauss_wts(shtns_cfg, double *wts);                    /* :185 */
 
 /* Legendre functions (shtns.h:227-228) */
 int legendre_sphPlm_array(shtns_cfg shtns, const int lmax, const int im, const double x, double *yl);
+/* m = 0: P_l0 and dP_l0/dtheta; m > 0: P_lm/sin(theta) and dP_lm/dtheta (shtns.h:233) */
+int legendre_sphPlm_deriv_array(shtns_cfg shtns, const int lmax, const int im, const double x, const double sint,
+                                double *yl, double *dyl);
+
+/* operators coupling degrees l-1 and l+1, mx of 2*nlm doubles (shtns.h:240-245) */
+void mul_ct_matrix(shtns_cfg, double *mx);
+void st_dt_matrix(shtns_cfg, double *mx);
+void SH_mul_mx(shtns_cfg, double *mx, cplx *Qlm, cplx *Rlm);
 
 /* scalar transforms (shtns.h:262-267).  spat_to_SH leaves Vr unchanged here
    (SHTns only promises nothing about it). */

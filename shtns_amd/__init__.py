@@ -172,6 +172,28 @@ class sht:
         q = np.ascontiguousarray(qlm, dtype=np.complex128)
         return _lib.load().SH_to_point(self._cfg, _ptr(q), float(cost), float(phi))
 
+    def mul_ct_matrix(self):
+        """Coefficients (2*nlm) of the multiplication by cos(theta), for SH_mul_mx (shtns_numpy.i:272-276)."""
+        mx = np.empty(2 * self.nlm)
+        _lib.load().mul_ct_matrix(self._cfg, _ptr(mx))
+        return mx
+
+    def st_dt_matrix(self):
+        """Coefficients (2*nlm) of sin(theta) d/dtheta, for SH_mul_mx (shtns_numpy.i:277-281)."""
+        mx = np.empty(2 * self.nlm)
+        _lib.load().st_dt_matrix(self._cfg, _ptr(mx))
+        return mx
+
+    def SH_mul_mx(self, mx, Qlm):
+        """R_lm = mx[2 lm] Q_{lm-1} + mx[2 lm + 1] Q_{lm+1} (shtns_numpy.i:651-659)."""
+        q = np.ascontiguousarray(Qlm, dtype=np.complex128)
+        m = np.ascontiguousarray(mx, dtype=np.float64)
+        if q.size != self.nlm or m.size != 2 * self.nlm:
+            raise RuntimeError("wrong array size.")
+        r = np.empty_like(q)
+        _lib.load().SH_mul_mx(self._cfg, _ptr(m), _ptr(q), _ptr(r))
+        return r
+
     def Zrotate(self, Qlm, alpha):
         """Rotate the field about the z axis by alpha (shtns_numpy.i:609-614)."""
         q = np.ascontiguousarray(Qlm, dtype=np.complex128)

@@ -23,6 +23,7 @@ EXPORTED_SYMBOLS = [
     "SHsphtor_to_spat_ml", "spat_to_SHsphtor_ml", "SHsph_to_spat_ml", "SHtor_to_spat_ml",
     "SHqst_to_spat_ml", "spat_to_SHqst_ml",
     "SH_to_point_cplx", "SH_to_grad_point", "SHqst_to_point", "SH_to_lat", "SHqst_to_lat", "SH_Zrotate",
+    "legendre_sphPlm_deriv_array", "mul_ct_matrix", "st_dt_matrix", "SH_mul_mx",
     "shtns_last_error", "shtns_last_error_string",
     # shtns_hip.h
     "cu_spat_to_SH", "cu_SH_to_spat", "cu_spat_to_SHsphtor", "cu_SHsphtor_to_spat",
@@ -115,6 +116,10 @@ def load():
         "SHqst_to_point": (None, [CFG, P, P, P, D, D, P, P, P]),
         "SH_to_lat": (None, [CFG, P, D, P, I, I, I]),
         "SH_Zrotate": (None, [CFG, P, D, P]),
+        "legendre_sphPlm_deriv_array": (I, [CFG, I, I, D, D, P, P]),
+        "mul_ct_matrix": (None, [CFG, P]),
+        "st_dt_matrix": (None, [CFG, P]),
+        "SH_mul_mx": (None, [CFG, P, P, P]),
         "SH_to_spat_ml": (None, [CFG, I, P, P, I]),
         "spat_to_SH_ml": (None, [CFG, I, P, P, I]),
         "SHsphtor_to_spat_ml": (None, [CFG, I, P, P, P, P, I]),

@@ -576,6 +576,65 @@ static void plm_deriv(int m, int ltop, double cost, std::vector<long double> &ys
     }
 }
 
+// Legendre functions and their theta derivatives, l = m..lmax (shtns.h:233,
+// sht_legendre.c:356-366): m = 0: yl = P_l0, dyl = dP_l0/dtheta; m > 0: yl = P_lm / sin(theta),
+// dyl = dP_lm/dtheta (finite at the poles).  Returns the first degree with a non-zero value.
+extern "C" int legendre_sphPlm_deriv_array(shtns_cfg c, const int lmax, const int im, const double x,
+                                           const double sint, double *yl, double *dyl) {
+    (void)sint;   // sin(theta) is recomputed in long double from x
+    const int m = im * c->mres;
+    if (im < 0 || im > c->mmax || lmax < m || lmax > c->lmax + 1) return -1;
+    std::vector<long double> ys, dy;
+    plm_deriv(m, lmax, x, ys, dy);
+    int lnz = lmax + 1;
+    for (int l = m; l <= lmax; ++l) {
+        yl[l - m] = (double)ys[l - m];
+        dyl[l - m] = (double)dy[l - m];
+        if (lnz > lmax && (yl[l - m] != 0.0 || dyl[l - m] != 0.0)) lnz = l;
+    }
+    return lnz;
+}
+
+// Operators coupling degrees l-1 and l+1 of one order (sht_func.c:450-500), orthonormal
+// functions: cos(theta) Y_l = a_l Y_{l+1} + a_{l-1} Y_{l-1}, a_l = sqrt(((l+1)^2 - m^2) / ((2l+1)(2l+3))),
+// and sin(theta) d/dtheta Y_l = l a_l Y_{l+1} - (l+1) a_{l-1} Y_{l-1}.  mx[2 lm] multiplies
+// Q_{l-1}, mx[2 lm + 1] multiplies Q_{l+1}; the coupling to degree lmax+1 is dropped.
+extern "C" void mul_ct_matrix(shtns_cfg c, double *mx) {
+    for (int im = 0; im <= c->mmax; ++im) {
+        const long m = (long)im * c->mres;
+        for (int l = (int)m; l <= c->lmax; ++l) {
+            const long lm = LiM(c, l, im);
+            const double lo = (l > m) ? sqrt((double)((long)l * l - m * m) / ((2.0 * l - 1) * (2.0 * l + 1))) : 0.0;
+            const double hi = (l < c->lmax)
+                                  ? sqrt((double)((long)(l + 1) * (l + 1) - m * m) / ((2.0 * l + 1) * (2.0 * l + 3)))
+                                  : 0.0;
+            mx[2 * lm] = lo;
+            mx[2 * lm + 1] = hi;
+        }
+    }
+}
+extern "C" void st_dt_matrix(shtns_cfg c, double *mx) {
+    mul_ct_matrix(c, mx);
+    for (unsigned lm = 0; lm < c->nlm; ++lm) {
+        const int l = c->li[lm];
+        mx[2 * lm] *= (double)(l - 1);
+        mx[2 * lm + 1] *= -(double)(l + 2);
+    }
+}
+// R_lm = mx[2 lm] Q_{lm-1} + mx[2 lm + 1] Q_{lm+1} (Rlm must differ from Qlm)
+extern "C" void SH_mul_mx(shtns_cfg c, double *mx, cplx *Qlm, cplx *Rlm) {
+    const double *q = reinterpret_cast<const double *>(Qlm);
+    double *r = reinterpret_cast<double *>(Rlm);
+    const long n = c->nlm;
+    for (long lm = 0; lm < n; ++lm) {
+        double re = 0.0, ii = 0.0;
+        if (lm > 0) { re += mx[2 * lm] * q[2 * lm - 2]; ii += mx[2 * lm] * q[2 * lm - 1]; }
+        if (lm + 1 < n) { re += mx[2 * lm + 1] * q[2 * lm + 2]; ii += mx[2 * lm + 1] * q[2 * lm + 3]; }
+        r[2 * lm] = re;
+        r[2 * lm + 1] = ii;
+    }
+}
+
 // std::complex<double> and C99 double complex share layout and return convention
 #pragma clang diagnostic push
 #pragma clang diagnostic ignored "-Wreturn-type-c-linkage"

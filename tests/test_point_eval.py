@@ -138,3 +138,40 @@ def test_zrotate(sizes):
     q2 = q.copy()
     _lib.load().SH_Zrotate(s._cfg, q2.ctypes.data, alpha, q2.ctypes.data)
     assert np.array_equal(q2, r)
+
+
+def test_legendre_deriv_array_matches_scipy():
+    s = shtns_amd.sht(30, 30)
+    for im in (0, 1, 7):
+        for x in (0.4, -0.9, 1.0):
+            th = np.arccos(x)
+            yl, dyl = np.empty(31 - im), np.empty(31 - im)
+            from shtns_amd import _lib
+            lnz = _lib.load().legendre_sphPlm_deriv_array(s._cfg, 30, im, x, np.sqrt(1 - x * x),
+                                                          yl.ctypes.data, dyl.ctypes.data)
+            assert lnz >= im
+            for l in range(im, 31):
+                P = np.real(sp.sph_harm_y(l, im, th, 0.0))
+                dP = np.real(dtheta_Y(l, im, th, 0.0))
+                if im == 0:
+                    assert abs(yl[l] - P) <= 1e-13
+                elif x != 1.0:
+                    assert abs(yl[l - im] * np.sin(th) - P) <= 1e-13
+                if x != 1.0 or im <= 1:
+                    assert abs(dyl[l - im] - dP) <= 1e-12 * (1 + l)
+
+
+@pytest.mark.parametrize("sizes", [(20, 20, 1), (24, 6, 2)])
+def test_mul_ct_and_st_dt(sizes):
+    """cos(theta) f and sin(theta) df/dtheta through SH_mul_mx, checked at points (degree
+    lmax is left empty so the products stay within lmax)."""
+    s = shtns_amd.sht(*sizes)
+    q = spectral_field(sizes[0], s.mmax, seed=13, mres=sizes[2])
+    q[s.l == s.lmax] = 0
+    rc = s.SH_mul_mx(s.mul_ct_matrix(), q)
+    rd = s.SH_mul_mx(s.st_dt_matrix(), q)
+    for cost, phi in POINTS:
+        f = s.SH_to_point(q, cost, phi)
+        assert abs(s.SH_to_point(rc, cost, phi) - cost * f) <= 1e-13
+        gt = s.SH_to_grad_point(q, q, cost, phi)[1]
+        assert abs(s.SH_to_point(rd, cost, phi) - np.sqrt(1 - cost * cost) * gt) <= 1e-12
