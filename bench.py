@@ -103,10 +103,10 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="after the timed loop, copy a 512 MiB buffer once (reference kernel of known "
                          "traffic for hardware-counter runs)")
-    ap.add_argument("--workload", choices=["pair", "uneven", "vector"], default="pair",
+    ap.add_argument("--workload", choices=["pair", "uneven", "vector", "cplx"], default="pair",
                     help="pair: the headline scalar pair at lmax=1023 (default); uneven: 256 transforms "
                          "of uneven lmax from one graph plus one lmax=4095 field; vector: spheroidal/"
-                         "toroidal pairs at lmax=511")
+                         "toroidal pairs at lmax=511; cplx: complex-valued scalar pairs at lmax=1023")
     ap.add_argument("--tune", action="store_true", help="autotune the kernel shapes (cushtns_tune) before timing")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -117,6 +117,8 @@ def main():
         return bench_uneven(args)
     if args.workload == "vector":
         return bench_vector(args)
+    if args.workload == "cplx":
+        return bench_cplx(args)
 
     import numpy as np
     import torch
@@ -398,6 +400,47 @@ def bench_vector(args):
         "config": {"workload": "32 (S,T) pairs, lmax=mmax=511, Gauss 768x1536 theta-contiguous", "lmax": L,
                    "nlat": nlat, "nphi": nphi, "batch": nb, "parallelism": "single GPU"},
         "legs": out}))
+
+
+def bench_cplx(args):
+    """16 complex-valued fields at lmax=mmax=1023 on 1536 x 3072 (SH_to_spat_cplx then
+    spat_cplx_to_SH, batched on the device): coefficients a_lm for -l <= m <= l, real and
+    imaginary parts ~ N(0, (l+1)^-2), seeded.  One complex pair is the work of two real pairs."""
+    import numpy as np
+    import torch
+    import shtns_amd
+    from shtns_amd import _lib
+    L, nlat, nphi, nb = 1023, 1536, 3072, 16
+    sh = shtns_amd.sht(L)
+    sh.set_grid(nlat, nphi, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+    rng = np.random.default_rng(6)
+    lfac = np.array([1.0 / (l + 1) for l in range(L + 1) for _ in range(2 * l + 1)])
+    A = torch.from_numpy((rng.standard_normal((nb, sh.nlm_cplx)) + 1j * rng.standard_normal((nb, sh.nlm_cplx)))
+                         * lfac).cuda()
+    for _ in range(args.warmup):
+        z = sh.synth_cplx(A)
+        a2 = sh.analys_cplx(z)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        z = sh.synth_cplx(A)
+        a2 = sh.analys_cplx(z)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    _lib.check_error("cplx")
+    err = float(torch.linalg.norm(a2 - A) / torch.linalg.norm(A))
+    nlm = (L + 1) * (L + 2) // 2
+    alg = 2 * 2 * (2 * nlat * nlm + fft_flops(nlat, nphi)) * nb     # two real fields per complex field
+    print(json.dumps({
+        "metric": "complex-valued SH_to_spat_cplx+spat_cplx_to_SH pairs/s at lmax=1023",
+        "value": round(nb * args.steps / el, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(el / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (numpy default_rng(6) a_lm, re/im ~ N(0,(l+1)^-2))",
+        "config": {"workload": "16 complex fields, lmax=mmax=1023, Gauss 1536x3072 theta-contiguous", "lmax": L,
+                   "nlat": nlat, "nphi": nphi, "batch": nb, "parallelism": "single GPU"},
+        "round_trip_rel_l2": err,
+        "fp64_frac_of_peak": round(alg * args.steps / el / 1e12 / FP64_PEAK_TFLOPS, 4)}))
 
 
 if __name__ == "__main__":
