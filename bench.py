@@ -111,7 +111,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is described in DESIGN.md but not built.", file=sys.stderr)
+              "sharding across GPUs is not built.", file=sys.stderr)
         sys.exit(2)
     if args.workload == "uneven":
         return bench_uneven(args)

@@ -1,5 +1,5 @@
 """GPU parity: the HIP transforms (through the C-ABI library) against the CPU oracle on the
-same seeded inputs.  FP64 bar: relative L2 error <= 1e-12 (north-star tolerance)."""
+same seeded inputs.  FP64 bar: relative L2 error <= 1e-12."""
 import numpy as np
 import pytest
 
