@@ -65,6 +65,18 @@ void cu_SHsphtor_to_spat_batch(shtns_cfg, cplx *Slm, cplx *Tlm, double *Vt, doub
 void cu_spat_to_SHsphtor_batch(shtns_cfg, double *Vt, double *Vp, cplx *Slm, cplx *Tlm, int ltr,
                                int nfields, long spat_dist, long spec_dist);
 
+/* No counterpart in SHTns.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): cu_SH_to_spat_batch
+ * with its work arrays (Fourier rows and pre-mixed coefficients) in a fixed 64 MiB device
+ * workspace that the library holds on each GPU until the process ends.  It allocates no
+ * device memory and creates no stream or event; the configuration must already hold its GPU
+ * tables (shtns_set_grid on a GPU, or cushtns_init_gpu).  A call needing more than 64 MiB
+ * (e.g. more than one field at lmax = 1023, nlat = 1536) fails with shtns_last_error() = 2
+ * and does nothing.  All calls on one GPU share the workspace: calls that overlap on one GPU
+ * (other streams, other threads) race and must be serialized by the caller. */
+void cu_SH_to_spat_batch_noalloc(shtns_cfg, cplx *Qlm, double *Vr, int ltr, int nfields, long spec_dist,
+                                 long spat_dist);
+
 /* Single-precision data, batched (additive; the unbatched _float calls above are SHTns's).
  * Arrays are FP32; the recurrence, seeds and Fourier transform run in FP64 and the
  * degree/latitude contraction at the configuration's precision (cushtns_set_precision;

@@ -37,6 +37,7 @@ EXPORTED_SYMBOLS = [
     "cu_spat_to_SH_float", "cu_SH_to_spat_float", "cu_SH_to_spat_float_batch", "cu_spat_to_SH_float_batch",
     "cushtns_set_precision", "cushtns_tune", "cushtns_tuning", "cushtns_set_tuning", "cushtns_batch_create", "cushtns_batch_arrays",
     "cushtns_batch_SH_to_spat", "cushtns_batch_spat_to_SH", "cushtns_batch_destroy",
+    "cu_SH_to_spat_batch_noalloc",
 ]
 
 class Cplx(ctypes.Structure):
@@ -170,6 +171,7 @@ def load():
         "cushtns_clone": (CFG, [CFG, P, P]),
         "cushtns_release_gpu": (None, [CFG]),
         "cu_SH_to_spat_batch": (None, [CFG, P, P, I, I, L, L]),
+        "cu_SH_to_spat_batch_noalloc": (None, [CFG, P, P, I, I, L, L]),
         "cu_spat_to_SH_batch": (None, [CFG, P, P, I, I, L, L]),
         "cu_SHsphtor_to_spat_batch": (None, [CFG, P, P, P, P, I, I, L, L]),
         "cu_spat_to_SHsphtor_batch": (None, [CFG, P, P, P, P, I, I, L, L]),

@@ -861,6 +861,10 @@ static int device_init(shtns_info *c) {
     launch_seed_tables(&d, L, mmax, mres, nl2, thr, nullptr);
     HIP_OK_RET(hipGetLastError(), -1);
     HIP_OK_RET(hipDeviceSynchronize(), -1);
+    // fixed workspace of cu_SH_to_spat_batch_noalloc: its address on this device (once per
+    // device; a failure only disables that entry point)
+    if (noalloc_scratch_setup(dev) != 0 && g_verbose)
+        fprintf(stderr, "[shtns_amd] no fixed workspace on device %d: cu_SH_to_spat_batch_noalloc disabled\n", dev);
     c->gpu_ready = true;
     return dev;
 }
@@ -987,8 +991,9 @@ struct ProfScope {
     int k;
     hipStream_t s;
     hipEvent_t a = nullptr, b = nullptr;
-    ProfScope(shtns_cfg c_, int k_, hipStream_t s_ = nullptr, bool use_s = false) : c(c_), k(k_), s(use_s ? s_ : c_->comp_stream) {
-        if (!c->prof) return;
+    ProfScope(shtns_cfg c_, int k_, hipStream_t s_ = nullptr, bool use_s = false, bool allowed = true)
+        : c(c_), k(k_), s(use_s ? s_ : c_->comp_stream) {
+        if (!c->prof || !allowed) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) { a = b = nullptr; return; }
         (void)hipEventRecord(a, s);
     }
@@ -1081,15 +1086,24 @@ static void *xscratch(shtns_cfg c, size_t bytes) {
 // V is double (vf == nullptr) or, for the FP32-data entry points, float: vf != nullptr, with
 // vstage an FP64 staging area of nfields * nspat doubles when the register FFT cannot take
 // the length (then the FFT writes vstage and a conversion kernel writes vf).
+// Scratch bytes of one synth_core call (Fourier rows F, then the pre-mixed coefficients).
+static size_t synth_scratch_bytes(shtns_cfg c, int nfields) {
+    const size_t fbytes = (size_t)nfields * (c->mmax + 1) * c->nlat * sizeof(double2);
+    return fbytes + synth_qmix_doubles(nfields, c->nsteps_total, c->tune_synth_ncg) * sizeof(double);
+}
+
+// ext != nullptr: a caller-provided workspace of at least synth_scratch_bytes(c, nfields)
+// bytes is used instead of the configuration's scratch; no stream, event or memory is then
+// created (no profiling, no overlapped chunks).
 static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
-                       double mpos_scale, int lat_weight, int prec, float *vf = nullptr, double *vstage = nullptr) {
+                       double mpos_scale, int lat_weight, int prec, float *vf = nullptr, double *vstage = nullptr,
+                       void *ext = nullptr) {
     if (nfields <= 0) return;
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
     const size_t fbytes = (size_t)nfields * four_dist * sizeof(double2);
-    const size_t qbytes = synth_qmix_doubles(nfields, c->nsteps_total, c->tune_synth_ncg) * sizeof(double);
-    char *base = (char *)scratch(c, fbytes + qbytes);
+    char *base = ext ? (char *)ext : (char *)scratch(c, synth_scratch_bytes(c, nfields));
     if (!base) return;
     double2 *F = (double2 *)base;
     double *qmix = (double *)(base + fbytes);
@@ -1103,17 +1117,18 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     const int ntiles = (c->nlat_2 + 63) / 64;
     const int chunk_tiles = 4;
     const int nchunks = (ntiles + chunk_tiles - 1) / chunk_tiles;
-    hipStream_t A = (overlap_enabled() && nchunks > 1 && !vf) ? aux_stream(c) : nullptr;
+    hipStream_t A = (overlap_enabled() && nchunks > 1 && !vf && !ext) ? aux_stream(c) : nullptr;
+    const bool prof = !ext;
     {
-        ProfScope ps(c, PK_LEG_SYNTH);
+        ProfScope ps(c, PK_LEG_SYNTH, nullptr, false, prof);
         launch_premix(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, qmix, c->nsteps_total, S);
     }
     if (!A) {
         {
-            ProfScope ps(c, PK_LEG_SYNTH);
+            ProfScope ps(c, PK_LEG_SYNTH, nullptr, false, prof);
             launch_legendre_synth_tiles(a, c->lmax, F, four_dist, qmix, c->nsteps_total, 0, ntiles, S);
         }
-        ProfScope ps(c, PK_FFT_SYNTH);
+        ProfScope ps(c, PK_FFT_SYNTH, nullptr, false, prof);
         if (!vf) {
             launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S, 0,
                              -1, c->tune_fft_synth);
@@ -1286,6 +1301,38 @@ extern "C" void cu_spat_to_SH_batch(shtns_cfg c, double *V, cplx *Q, int ltr, in
     clear_error();
     if (!ready(c, "cu_spat_to_SH")) return;
     anal_core(c, V, Q, ltr, nf, spat_dist, spec_dist, 1.0, 1, prec_f64(c));
+}
+// Additive, no SHTns counterpart: cu_SH_to_spat_batch with the Fourier rows and pre-mixed
+// coefficients in the library's fixed per-GPU workspace (kNoallocScratchBytes), for callers
+// that cannot allocate device memory (HIP graph capture, tight serving loops).  It makes no
+// device allocation, stream or event.  The configuration must already hold its GPU tables.
+// Every call on one GPU shares that workspace: calls that overlap on one GPU race.
+extern "C" void cu_SH_to_spat_batch_noalloc(shtns_cfg c, cplx *Q, double *V, int ltr, int nf, long spec_dist,
+                                            long spat_dist) {
+    clear_error();
+    if (c == nullptr || c->ct == nullptr || !c->gpu_ready) {
+        set_error(3, "cu_SH_to_spat_batch_noalloc: configuration has no GPU tables (call cushtns_init_gpu first)");
+        return;
+    }
+    if (nf < 0 || (nf > 0 && (spec_dist < (long)c->nlm || spat_dist < (long)c->nspat))) {
+        set_error(1, "cu_SH_to_spat_batch_noalloc: bad batch (nfields=%d, spec_dist=%ld, spat_dist=%ld)", nf, spec_dist,
+                  spat_dist);
+        return;
+    }
+    if (nf == 0) return;
+    const size_t need = synth_scratch_bytes(c, nf);
+    if (need > kNoallocScratchBytes) {
+        set_error(2, "cu_SH_to_spat_batch_noalloc: needs %zu bytes of workspace, above the fixed %zu bytes "
+                     "(fewer fields per call, or cu_SH_to_spat_batch)", need, kNoallocScratchBytes);
+        return;
+    }
+    (void)hipSetDevice(c->device);
+    void *ws = noalloc_scratch_ptr(c->device);
+    if (ws == nullptr) {
+        set_error(3, "cu_SH_to_spat_batch_noalloc: no fixed workspace on device %d", c->device);
+        return;
+    }
+    synth_core(c, Q, V, ltr, nf, spec_dist, spat_dist, 1.0, 0, prec_f64(c), nullptr, nullptr, ws);
 }
 extern "C" void cu_SH_to_spat(shtns_cfg c, cplx *Q, double *V, int ltr) {
     cu_SH_to_spat_batch(c, Q, V, ltr, 1, c->nlm, c->nspat);

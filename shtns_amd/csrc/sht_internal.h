@@ -120,6 +120,14 @@ void clear_error();
 // Contraction precisions of the Legendre kernels (SHT_PREC_* in shtns_hip.h)
 constexpr int kPrecF64 = 0, kPrecF32 = 1, kPrecSplit = 2;
 
+// Fixed device workspace of cu_SH_to_spat_batch_noalloc (sht_noalloc.hip): one array per
+// GPU, kept until the process ends.  noalloc_scratch_setup resolves the array's device
+// address on device `dev` (which must be current) once per device; noalloc_scratch_ptr
+// returns it (nullptr if the set-up failed or never ran on that device).
+constexpr size_t kNoallocScratchBytes = (size_t)64 << 20;
+int noalloc_scratch_setup(int dev);
+void *noalloc_scratch_ptr(int dev);
+
 // Fourier buffer layout: F[field][im][nlat] complex (north to south).
 // qmix: scratch of synth_qmix_doubles(nfields, nsteps_total) doubles (pre-mixed coefficients)
 void launch_legendre_synth(const LegendreArgs &a, const double2 *Qlm, long spec_dist, int spec_ltop,
