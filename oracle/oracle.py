@@ -48,6 +48,8 @@ def _load():
         lib.orc_SHsphtor_to_spat.argtypes = [P, P, P, P, P, I]
         lib.orc_spat_to_SHsphtor.argtypes = [P, P, P, P, P, I]
         lib.orc_fft.argtypes = [P, I, I]
+        lib.orc_set_polar_opt.argtypes = [P, D]
+        lib.orc_polar_tm.argtypes = [P, P]
         _lib = lib
     return _lib
 
@@ -93,6 +95,16 @@ class Oracle:
         if getattr(self, "_h", None):
             _load().orc_destroy(self._h)
             self._h = None
+
+    def set_polar_opt(self, eps):
+        """SHTns's polar optimization threshold (sht_init.c:522-557); 0 disables it."""
+        _load().orc_set_polar_opt(self._h, float(eps))
+
+    def polar_tm(self):
+        """First latitude index of each order im = 0..mmax used by the transforms."""
+        tm = np.empty(self.mmax + 1, dtype=np.int32)
+        _load().orc_polar_tm(self._h, _p(tm))
+        return tm
 
     def idx(self, l, m):
         im = m // self.mres

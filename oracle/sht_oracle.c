@@ -20,6 +20,8 @@
  *                       Y_lm/sin(theta); m=0 via dY_l0/dtheta (kernel_SH_to_spat.gen.c:94-202)
  *   vector analysis     sht_private.h:296-337 (SH_2scal_to_vect with mx_van = 2*ct + st_dt,
  *                       sht_init.c:1382-1386), m=0 via dY/dtheta (kernel_spat_to_SH.gen.c:89-192)
+ *   orc_set_polar_opt() sht_init.c:522-557 (PolarOptimize), applied as in
+ *                       kernel_SH_to_spat.gen.c:235-257 and kernel_spat_to_SH.gen.c:198-201
  *
  * This restatement uses the classic three-term recurrence for every m (as SHTns's
  * GPU kernels do, sht_gpu_kernels.cu:800-833), not the Ishioka variant of the CPU
@@ -46,6 +48,7 @@ struct orc_cfg {
     int mtab_n;               /* tables for m = 0..mtab_n-1 (only needed ones filled) */
     double **seed;            /* seed[m] = a_mm (1 value) */
     double **ab;              /* ab[m][2*(l-m)] = b_l, [2*(l-m)+1] = a_l ; l = m..lmax+1 */
+    int *tm;                  /* [mmax+1] first latitude pair of order im (polar optimization) */
 };
 
 static int g_threads = 0;
@@ -196,13 +199,41 @@ orc_cfg *orc_create(int lmax, int mmax, int mres, int nlat, int nphi) {
     c->ab = (double **)calloc(c->mtab_n, sizeof(double *));
     for (int im = 0; im <= mmax; ++im) fill_m_table_(c, im * mres);
     if (lmax >= 1) fill_m_table_(c, 1);   /* m=1 functions give dY_l0/dtheta */
+    c->tm = (int *)calloc(mmax + 1, sizeof(int));
     return c;
 }
 
 void orc_destroy(orc_cfg *c) {
     if (!c) return;
     for (int m = 0; m < c->mtab_n; ++m) { free(c->ab[m]); free(c->seed[m]); }
-    free(c->ab); free(c->seed); free(c->ct); free(c);
+    free(c->ab); free(c->seed); free(c->ct); free(c->tm); free(c);
+}
+
+/* ---- polar optimization (sht_init.c:522-557): tm[im] = first latitude index it (from the
+ *      pole) with max_l |Y_l^m(ct[it])| >= eps, searched from tm[im-1]; tm[0] = 0.  The
+ *      transforms start order im at latitude (tm[im]/2)*2, the even start of SHTns's
+ *      kernels without SIMD (kernel_SH_to_spat.gen.c:235-240, kernel_spat_to_SH.gen.c:198-201). */
+void orc_set_polar_opt(orc_cfg *c, double eps) {
+    for (int im = 0; im <= c->mmax; ++im) c->tm[im] = 0;
+    if (!(eps > 0.0)) return;
+    double *y = (double *)malloc(sizeof(double) * (c->lmax + 2));
+    int it = 0;
+    for (int im = 1; im <= c->mmax; ++im) {
+        const int m = im * c->mres;
+        while (it < c->nlat_2) {
+            leg_column_m_(c, m, c->ct[it], c->st[it], c->lmax, y);
+            double v = 0.0;
+            for (int l = m; l <= c->lmax; ++l) if (fabs(y[l - m]) > v) v = fabs(y[l - m]);
+            if (v >= eps) break;
+            ++it;
+        }
+        c->tm[im] = (it / 2) * 2;
+    }
+    free(y);
+}
+
+void orc_polar_tm(const orc_cfg *c, int *tm) {
+    for (int im = 0; im <= c->mmax; ++im) tm[im] = c->tm[im];
 }
 
 void orc_grid(const orc_cfg *c, double *ct, double *st, double *w) {
@@ -313,7 +344,7 @@ void orc_SH_to_spat(const orc_cfg *c, const double *Q, double *Vr, int ltr) {
         const int m = im * c->mres;
         double *y = (double *)malloc(sizeof(double) * (ltr - m + 2));
         const double *Ql = Q + 2 * lm_index(c, 0, im);
-        for (int k = 0; k < nl2; ++k) {
+        for (int k = c->tm[im]; k < nl2; ++k) {
             leg_column_m_(c, m, c->ct[k], c->st[k], ltr, y);
             double er = 0, ei = 0, orr = 0, oi = 0;
             for (int l = m; l <= ltr; ++l) {
@@ -346,7 +377,7 @@ void orc_spat_to_SH(const orc_cfg *c, const double *Vr, double *Q, int ltr) {
         const int m = im * c->mres;
         double *y = (double *)malloc(sizeof(double) * (ltr - m + 2));
         double *acc = (double *)calloc(2 * (size_t)(ltr - m + 1), sizeof(double));
-        for (int k = 0; k < nl2; ++k) {
+        for (int k = c->tm[im]; k < nl2; ++k) {
             leg_column_m_(c, m, c->ct[k], c->st[k], ltr, y);
             const double wk = c->w[k] * wfac;
             const double *Fn = F + 2 * ((size_t)k * mm1 + im);
@@ -422,7 +453,7 @@ void orc_SHsphtor_to_spat(const orc_cfg *c, const double *S, const double *T, do
             }
         }
         const int ltop = (m == 0) ? ltr : ltr + 1;
-        for (int k = 0; k < nl2; ++k) {
+        for (int k = c->tm[im]; k < nl2; ++k) {
             leg_column_m_(c, mcol, c->ct[k], c->st[k], ltop, y);
             const double s1 = (m == 0) ? 1.0 : 1.0 / c->st[k];
             double ver = 0, vei = 0, vor = 0, voi = 0, wer = 0, wei = 0, wor = 0, woi = 0;
@@ -469,7 +500,7 @@ void orc_spat_to_SHsphtor(const orc_cfg *c, const double *Vt, const double *Vp, 
         double *v = (double *)calloc(2 * (size_t)(n + 1), sizeof(double));
         double *w = (double *)calloc(2 * (size_t)(n + 1), sizeof(double));
         double *y = (double *)malloc(sizeof(double) * (n + 1));
-        for (int k = 0; k < nl2; ++k) {
+        for (int k = c->tm[im]; k < nl2; ++k) {
             leg_column_m_(c, mcol, c->ct[k], c->st[k], ltop, y);
             const double wk = c->w[k] * wfac * ((m == 0) ? 1.0 : 1.0 / c->st[k]);
             const double *Fn = Ft + 2 * ((size_t)k * mm1 + im), *Fs = Ft + 2 * ((size_t)(nlat - 1 - k) * mm1 + im);

@@ -35,6 +35,12 @@ void orc_grid(const orc_cfg *c, double *ct, double *st, double *w);
 /* Normalized associated Legendre functions Y_l^m(x), l=m..ltop (ltop<=lmax+1), m=im*mres. */
 void orc_legendre(const orc_cfg *c, int im, double x, double sint, int ltop, double *yl);
 
+/* Polar optimization of SHTns (PolarOptimize, sht_init.c:522-557): with eps > 0, order im
+ * skips the latitude pairs below tm[im] (rounded down to even); eps <= 0 (the default after
+ * orc_create) skips nothing.  orc_polar_tm copies the mmax+1 start indices. */
+void orc_set_polar_opt(orc_cfg *c, double eps);
+void orc_polar_tm(const orc_cfg *c, int *tm);
+
 /* Transforms; ltr<0 means lmax.  Inputs are never modified. */
 void orc_SH_to_spat(const orc_cfg *c, const double *Qlm, double *Vr, int ltr);
 void orc_spat_to_SH(const orc_cfg *c, const double *Vr, double *Qlm, int ltr);

@@ -302,26 +302,64 @@ class sht:
         if tuple(v.shape[-2:]) != tuple(self.spat_shape):
             raise RuntimeError("spatial array has wrong shape.")
 
+    # host arrays handed to the C API (shtns_numpy.i:60-90 check_spatial / check_spectral):
+    # numpy, float64 / complex128, C-contiguous, exactly nspat / nlm (or nlm_cplx) elements
+    def _host_spat(self, i, a, dtype=np.float64):
+        self._need_grid()
+        if not isinstance(a, np.ndarray):
+            raise TypeError(f"arg #{i} : numpy array expected.")
+        if a.dtype != dtype:
+            raise TypeError(f"arg #{i} : spatial array must consist of {np.dtype(dtype).name}.")
+        if not a.flags.c_contiguous:
+            raise RuntimeError(f"arg #{i} : spatial array not contiguous. Use 'b=a.copy()' to copy a to a contiguous array b.")
+        if a.size != self.nspat:
+            raise RuntimeError(f"arg #{i} : spatial array has wrong size")
+
+    def _host_spec(self, i, a, size=None):
+        if not isinstance(a, np.ndarray):
+            raise RuntimeError(f"arg #{i} : numpy array expected.")
+        if a.dtype != np.complex128:
+            raise RuntimeError(f"arg #{i} : spectral array must consist of complex float. Create with: 'sh.spec_array()'")
+        if not a.flags.c_contiguous:
+            raise RuntimeError(f"arg #{i} : spectral array not contiguous. Use .copy() to copy to a contiguous array.")
+        if a.size != (self.nlm if size is None else size):
+            raise RuntimeError(f"arg #{i} : spectral array has wrong size")
+
+    def _host_args(self, spec=(), spat=(), cplx=False):
+        """Validate the spectral then spatial host arrays of one call (argument numbers as in
+        the call)."""
+        i = 1
+        for a in spec:
+            self._host_spec(i, a, self.nlm_cplx if cplx else None)
+            i += 1
+        for a in spat:
+            self._host_spat(i, a, np.complex128 if cplx else np.float64)
+            i += 1
+
     # ---- numpy (host) transforms, SHTns names ----
     def SH_to_spat(self, Qlm, Vr, ltr=-1):
         self._need_grid()
+        self._host_args(spec=(Qlm,), spat=(Vr,))
         lib = _lib.load()
         lib.SH_to_spat_l(self._cfg, _ptr(Qlm), _ptr(Vr), self.lmax if ltr < 0 else ltr)
         _lib.check_error("SH_to_spat")
 
     def spat_to_SH(self, Vr, Qlm, ltr=-1):
         self._need_grid()
+        self._host_args(spec=(Qlm,), spat=(Vr,))
         lib = _lib.load()
         lib.spat_to_SH_l(self._cfg, _ptr(Vr), _ptr(Qlm), self.lmax if ltr < 0 else ltr)
         _lib.check_error("spat_to_SH")
 
     def SHsphtor_to_spat(self, Slm, Tlm, Vt, Vp, ltr=-1):
         self._need_grid()
+        self._host_args(spec=(Slm, Tlm), spat=(Vt, Vp))
         _lib.load().SHsphtor_to_spat_l(self._cfg, _ptr(Slm), _ptr(Tlm), _ptr(Vt), _ptr(Vp), self.lmax if ltr < 0 else ltr)
         _lib.check_error("SHsphtor_to_spat")
 
     def spat_to_SHsphtor(self, Vt, Vp, Slm, Tlm, ltr=-1):
         self._need_grid()
+        self._host_args(spec=(Slm, Tlm), spat=(Vt, Vp))
         _lib.load().spat_to_SHsphtor_l(self._cfg, _ptr(Vt), _ptr(Vp), _ptr(Slm), _ptr(Tlm), self.lmax if ltr < 0 else ltr)
         _lib.check_error("spat_to_SHsphtor")
 
@@ -329,6 +367,7 @@ class sht:
         """3-component synthesis: radial Qlm -> Vr, tangential Slm, Tlm -> Vt, Vp
         (shtns_numpy.i:376-380)."""
         self._need_grid()
+        self._host_args(spec=(Qlm, Slm, Tlm), spat=(Vr, Vt, Vp))
         _lib.load().SHqst_to_spat_l(self._cfg, _ptr(Qlm), _ptr(Slm), _ptr(Tlm), _ptr(Vr), _ptr(Vt), _ptr(Vp),
                                     self.lmax if ltr < 0 else ltr)
         _lib.check_error("SHqst_to_spat")
@@ -336,6 +375,7 @@ class sht:
     def spat_to_SHqst(self, Vr, Vt, Vp, Qlm, Slm, Tlm, ltr=-1):
         """3-component analysis (shtns_numpy.i:371-375)."""
         self._need_grid()
+        self._host_args(spec=(Qlm, Slm, Tlm), spat=(Vr, Vt, Vp))
         _lib.load().spat_to_SHqst_l(self._cfg, _ptr(Vr), _ptr(Vt), _ptr(Vp), _ptr(Qlm), _ptr(Slm), _ptr(Tlm),
                                     self.lmax if ltr < 0 else ltr)
         _lib.check_error("spat_to_SHqst")
@@ -343,12 +383,14 @@ class sht:
     def SH_to_spat_cplx(self, alm, z):
         """Complex-valued field: alm[LM_cplx(l, m)], -l <= m <= l -> z (shtns_numpy.i:333-337)."""
         self._need_grid()
+        self._host_args(spec=(alm,), spat=(z,), cplx=True)
         _lib.load().SH_to_spat_cplx(self._cfg, _ptr(alm), _ptr(z))
         _lib.check_error("SH_to_spat_cplx")
 
     def spat_cplx_to_SH(self, z, alm):
         """Complex-valued field z -> alm[LM_cplx(l, m)] (shtns_numpy.i:328-332)."""
         self._need_grid()
+        self._host_args(spec=(alm,), spat=(z,), cplx=True)
         _lib.load().spat_cplx_to_SH(self._cfg, _ptr(z), _ptr(alm))
         _lib.check_error("spat_cplx_to_SH")
 
@@ -361,12 +403,14 @@ class sht:
     def SHsphtor_to_spat_cplx(self, Slm, Tlm, Vt, Vp):
         """Complex-valued 2D vector field from Slm, Tlm in the LM_cplx layout (shtns_numpy.i:365-369)."""
         self._need_grid()
+        self._host_args(spec=(Slm, Tlm), spat=(Vt, Vp), cplx=True)
         _lib.load().SHsphtor_to_spat_cplx(self._cfg, _ptr(Slm), _ptr(Tlm), _ptr(Vt), _ptr(Vp))
         _lib.check_error("SHsphtor_to_spat_cplx")
 
     def spat_cplx_to_SHsphtor(self, Vt, Vp, Slm, Tlm):
         """Complex-valued 2D vector field -> Slm, Tlm (shtns_numpy.i:360-364)."""
         self._need_grid()
+        self._host_args(spec=(Slm, Tlm), spat=(Vt, Vp), cplx=True)
         _lib.load().spat_cplx_to_SHsphtor(self._cfg, _ptr(Vt), _ptr(Vp), _ptr(Slm), _ptr(Tlm))
         _lib.check_error("spat_cplx_to_SHsphtor")
 
@@ -476,6 +520,7 @@ class sht:
         if n not in (1, 2, 3):
             raise RuntimeError("1, 2 or 3 spectral arrays required.")
         if _is_torch(arg[0]):
+            self._same_tensors(arg)
             if n == 3:   # radial scalar, then the tangential vector pair (one stream)
                 return (self._synth_torch(arg[:1], ltr),) + tuple(self._synth_torch(arg[1:], ltr))
             return self._synth_torch(arg, ltr)
@@ -505,6 +550,7 @@ class sht:
         if n not in (1, 2, 3):
             raise RuntimeError("1, 2 or 3 spatial arrays required.")
         if _is_torch(arg[0]):
+            self._same_tensors(arg)
             if n == 3:
                 return (self._analys_torch(arg[:1], ltr),) + tuple(self._analys_torch(arg[1:], ltr))
             return self._analys_torch(arg, ltr)
@@ -525,9 +571,27 @@ class sht:
         return s, t
 
     # ---- torch (device) transforms ----
+    @staticmethod
+    def _same_tensors(arg):
+        """Every tensor of one call must be a torch tensor of the first one's shape, dtype and
+        device: the batch size and strides handed to the kernels come from the first."""
+        a0 = arg[0]
+        for i, a in enumerate(arg[1:], start=2):
+            if not _is_torch(a) or a.shape != a0.shape or a.dtype != a0.dtype or a.device != a0.device:
+                raise RuntimeError(f"arg #{i} : every tensor of a call must have the shape, dtype and device of the "
+                                   f"first ({tuple(a0.shape)}, {a0.dtype}, {a0.device})")
+
     def _stream(self, t):
+        """Run this configuration's next kernels on torch's current stream.  All calls on one
+        configuration share its device scratch, so when the stream changes the new stream
+        first waits for the work queued on the previous one (no overlap, no race)."""
         import torch
-        s = torch.cuda.current_stream(t.device).cuda_stream
+        cur = torch.cuda.current_stream(t.device)
+        prev = getattr(self, "_last_stream", None)
+        if prev is not None and prev != cur:
+            cur.wait_stream(prev)
+        self._last_stream = cur
+        s = cur.cuda_stream
         _lib.load().cushtns_set_streams(self._cfg, ctypes.c_void_p(s), ctypes.c_void_p(s))
 
     def _torch_spec(self, q, allow_f32=False):

@@ -133,33 +133,44 @@ static void fejer1_grid(double *x, double *st, double *w, int n) {
     }
 }
 
-// Gauss-Legendre nodes x (descending, north first) and weights (sum 2), Newton iteration
-// from the same initial guess as SHTns (sht_legendre.c:763-811).
+// Gauss-Legendre nodes x_k = cos(theta_k) (descending, north first) and weights (sum 2).
+// Each root of P_n is found by Newton's method on the colatitude, in long double:
+//   f(theta) = P_n(cos theta),  f'(theta) = n (cos(theta) P_n - P_{n-1}) / sin(theta),
+// from Tricomi's asymptotic first guess theta = pi (4k+3)/(4n+2) (k-th root from the north
+// pole), with P_n and P_{n-1} from the three-term recurrence; it stops when the step falls
+// below 1e-18 rad.  The weights are w_k = 2 sin^2(theta_k) / (n P_{n-1}(x_k))^2 and sin(theta)
+// comes from theta itself, so the polar values keep full relative precision.
 static void gauss_nodes(double *x, double *st, double *w, int n) {
-    const double eps = 2.3e-16;
-    for (long i = 0; i < n / 2; ++i) {
-        double z = (1.0 - (n - 1.) / (8. * n * n * n)) * cos((M_PI * (4 * i + 3)) / (4 * n + 2));
-        double z1, pp, p1, p2;
-        int k = 10;
-        do {
-            p1 = z;
-            p2 = 1.0;
-            for (long l = 2; l <= n; ++l) {
-                const double p3 = p2;
-                p2 = p1;
-                p1 = ((2 * l - 1) * z * p2 - (l - 1) * p3) / l;
+    const long double pi = 3.141592653589793238462643383279502884L;
+    for (int k = 0; k < n / 2; ++k) {
+        long double th = pi * (4 * k + 3) / (4.0L * n + 2), pn = 0, pn1 = 0;
+        for (int it = 0; it < 30; ++it) {
+            const long double c = cosl(th), s = sinl(th);
+            long double a = 1.0L, b = c;            // P_{l-1}, P_l
+            for (int l = 1; l < n; ++l) {
+                const long double nx = ((2 * l + 1) * c * b - l * a) / (l + 1);
+                a = b;
+                b = nx;
             }
-            pp = n * (p2 - z * p1);
-            z1 = z;
-            z -= p1 * (1. - z * z) / pp;
-        } while ((fabs(z - z1) > (z1 + z) * 0.5 * eps) && (--k > 0));
-        const double s2 = 1. - z * z;
-        x[i] = z;
-        x[n - 1 - i] = -z;
-        w[i] = 2.0 * s2 / (pp * pp);
-        w[n - 1 - i] = w[i];
-        st[i] = sqrt(s2);
-        st[n - 1 - i] = st[i];
+            pn = b;
+            pn1 = a;
+            const long double step = pn * s / (n * (c * pn - pn1));
+            th -= step;
+            if (fabsl(step) < 1e-18L) break;
+        }
+        const long double c = cosl(th), s = sinl(th);
+        long double a = 1.0L, b = c;
+        for (int l = 1; l < n; ++l) {
+            const long double nx = ((2 * l + 1) * c * b - l * a) / (l + 1);
+            a = b;
+            b = nx;
+        }
+        pn1 = a;
+        const long double wk = 2.0L * s * s / ((n * pn1) * (n * pn1));
+        x[k] = (double)c;
+        x[n - 1 - k] = (double)-c;
+        st[k] = st[n - 1 - k] = (double)s;
+        w[k] = w[n - 1 - k] = (double)wk;
     }
 }
 
@@ -238,12 +249,15 @@ static void free_device(shtns_info *c) {
     (void)hipSetDevice(c->device);
     if (c->clone_of == nullptr) {
         DeviceTables &d = c->d;
-        void *ptrs[] = {d.x, d.x2, d.st, d.wq, d.step_ofs, d.ab, d.pm, d.i0, d.z0, d.roots, d.fft_pos};
+        void *ptrs[] = {d.x, d.x2, d.st, d.wq, d.step_ofs, d.ab, d.pm, d.tm, d.i0, d.z0, d.roots, d.fft_pos};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     }
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->xscratch) (void)hipFree(c->xscratch);
+    for (auto &sl : c->hstage)
+        if (sl.first) (void)hipFree(sl.first);
+    c->hstage.clear();
     c->xscratch = nullptr;
     c->xscratch_bytes = 0;
     if (c->own_streams) {
@@ -259,9 +273,30 @@ static void free_device(shtns_info *c) {
     c->gpu_ready = false;
 }
 
-extern "C" void cushtns_release_gpu(shtns_cfg c) { free_device(c); }
+// A clone (cushtns_clone, and every configuration of an uneven batch) borrows its source's
+// device tables, and batches capture them in HIP graphs.  Freeing or rebuilding the source's
+// grid while a clone lives would leave those pointers dangling, so it is refused.
+static bool has_live_clones(shtns_cfg c) {
+    std::lock_guard<std::mutex> g(g_mutex);
+    for (shtns_info *p = g_cfg_list; p; p = p->next)
+        if (p->clone_of == c) return true;
+    return false;
+}
+
+extern "C" void cushtns_release_gpu(shtns_cfg c) {
+    clear_error();
+    if (has_live_clones(c)) {
+        set_error(1, "cushtns_release_gpu: configuration still has live clones (or batches); destroy them first");
+        return;
+    }
+    free_device(c);
+}
 
 extern "C" void shtns_unset_grid(shtns_cfg c) {
+    if (has_live_clones(c)) {
+        set_error(1, "shtns_unset_grid: configuration still has live clones (or batches); destroy them first");
+        return;
+    }
     prof_clear(c);
     free_device(c);
     delete[] c->ct;
@@ -380,6 +415,30 @@ static void fft_positions(const FftPlan &plan, std::vector<int> &pos) {
 
 static int device_init(shtns_info *c);
 
+// SHTns's polar optimization (sht_init.c:522-557): tm[im] is the first latitude (from the
+// pole) where max_l |Y_l^m(cos theta)| >= eps, searched upward from tm[im-1] (tm[0] = 0).
+// The transforms skip latitude pairs below it, from an even index as SHTns's kernels do
+// without SIMD (kernel_SH_to_spat.gen.c:235-240, kernel_spat_to_SH.gen.c:198-201): rows of
+// order im there are zero in synthesis and left out of the analysis.  Slot mmax+1 (the m=1
+// functions used by vector transforms at m=0) is never skipped, as SHTns's m=0.
+static void polar_optimize(shtns_cfg c, double eps) {
+    c->tm.assign(c->mmax + 2, 0);
+    if (!(eps > 0.0)) return;
+    std::vector<double> y(c->lmax + 1);
+    int it = 0;
+    for (int im = 1; im <= c->mmax; ++im) {
+        const int m = im * c->mres;
+        while (it < c->nlat_2) {
+            legendre_sphPlm_array(c, c->lmax, im, c->ct[it], y.data());
+            double v = 0.0;
+            for (int l = m; l <= c->lmax; ++l) v = std::max(v, fabs(y[l - m]));
+            if (v >= eps) break;
+            ++it;
+        }
+        c->tm[im] = (it >> 1) * 2;
+    }
+}
+
 extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double eps, int nl_order, int *nlat, int *nphi) {
     clear_error();
     const int grid = (int)flags & 255;
@@ -387,6 +446,14 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     const bool regular = (grid == sht_reg_fast || grid == sht_reg_dct);   // Fejer grid, no poles
     if (!regular && grid != sht_gauss && grid != sht_auto && grid != sht_quick_init && grid != sht_gauss_fly) {
         set_error(1, "grid type %d is not implemented (Gauss and regular grids without poles are)", grid);
+        return 0;
+    }
+    if (c->clone_of != nullptr) {
+        set_error(1, "shtns_set_grid: a clone shares the grid of its source configuration");
+        return 0;
+    }
+    if (has_live_clones(c)) {
+        set_error(1, "shtns_set_grid: configuration still has live clones (or batches); destroy them first");
         return 0;
     }
     if (nl_order <= 0) nl_order = 1;
@@ -450,6 +517,7 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
         for (unsigned i = 0; i < c->nlat; ++i) c->ct[i] = -c->ct[i];
     c->wg.assign(w.begin(), w.begin() + c->nlat_2);
     c->d.plan = plan;
+    polar_optimize(c, eps);
 
     // GPU tables (a config is usable for queries without a GPU; transforms then fail)
     if (g_gpu_request >= 0) {
@@ -844,7 +912,8 @@ static int device_init(shtns_info *c) {
     if (upload(&d.x, x.data(), nl2) || upload(&d.x2, x2.data(), nl2) || upload(&d.st, st.data(), nl2) ||
         upload(&d.wq, wq.data(), nl2) || upload(&d.step_ofs, ofs.data(), ofs.size()) ||
         upload(&d.ab, ab.data(), ab.size()) || upload(&d.pm, pm.data(), pm.size()) ||
-        upload(&d.roots, roots.data(), roots.size()) || upload(&d.fft_pos, pos.data(), pos.size())) {
+        upload(&d.roots, roots.data(), roots.size()) || upload(&d.fft_pos, pos.data(), pos.size()) ||
+        upload(&d.tm, c->tm.data(), c->tm.size())) {
         set_error(2, "device table allocation failed");
         c->gpu_ready = true;
         free_device(c);
@@ -857,8 +926,10 @@ static int device_init(shtns_info *c) {
         free_device(c);
         return -1;
     }
-    const double thr = (c->eps > 1e-280) ? c->eps : 1e-280;
-    launch_seed_tables(&d, L, mmax, mres, nl2, thr, nullptr);
+    // recurrences start where the functions exceed 1e-280 (their products with any
+    // coefficient are then below double resolution of the result); the polar threshold
+    // eps acts through the per-order latitude skip c->tm only, as in SHTns
+    launch_seed_tables(&d, L, mmax, mres, nl2, 1e-280, nullptr);
     HIP_OK_RET(hipGetLastError(), -1);
     HIP_OK_RET(hipDeviceSynchronize(), -1);
     // fixed workspace of cu_SH_to_spat_batch_noalloc: its address on this device (once per
@@ -1419,17 +1490,29 @@ extern "C" void cu_spat_to_SH_adjoint_batch(shtns_cfg c, cplx *Q, double *V, int
 }
 
 // ---- host-pointer API: copy in, run on the GPU, copy out (synchronous) ----
+// The device copies live in the configuration's staging slots (c->hstage): the i-th array a
+// call stages reuses slot i, so repeated calls allocate nothing.  A slot is regrown (after
+// the compute stream has drained) only when a call needs more than it holds.  Host arrays
+// from shtns_malloc (pinned) copy at full link speed; pageable ones are staged by HIP.
 struct HostStage {
     shtns_cfg c;
-    std::vector<void *> bufs;
-    ~HostStage() {
-        for (void *p : bufs) (void)hipFree(p);
-    }
+    size_t next = 0;
     void *dev(size_t bytes) {
-        void *p = nullptr;
-        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
-        bufs.push_back(p);
-        return p;
+        auto &slots = c->hstage;
+        if (next == slots.size()) slots.emplace_back(nullptr, 0);
+        auto &sl = slots[next];
+        if (sl.second < bytes) {
+            if (sl.first) {
+                (void)hipStreamSynchronize(c->comp_stream);
+                (void)hipFree(sl.first);
+            }
+            sl = {nullptr, 0};
+            void *p = nullptr;
+            if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+            sl = {p, bytes};
+        }
+        ++next;
+        return sl.first;
     }
 };
 
@@ -1770,8 +1853,10 @@ static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g
     const long double amm = seed_amm(m);
     const double lseed = (double)log2l(fabsl(amm)), sgn = amm < 0 ? -1.0 : 1.0;
     if (g.nl > 0) {
-        if (vector) launch_ml_vtable(g.ct, g.sn, nlat, g.m, g.nl, g.ab, lseed, sgn, g.T, g.D, s);
-        else launch_ml_table(g.ct, g.sn, nlat, m, g.nl, g.ab, lseed, sgn, g.T, s);
+        const int ktm = c->tm[im];   // SHTns zeroes these polar rows in its fixed-m kernels too
+                                     // (fly_SH_to_spat.gen.c:335-345)
+        if (vector) launch_ml_vtable(g.ct, g.sn, nlat, g.m, g.nl, g.ab, lseed, sgn, ktm, g.T, g.D, s);
+        else launch_ml_table(g.ct, g.sn, nlat, m, g.nl, g.ab, lseed, sgn, ktm, g.T, s);
     }
     return true;
 }

@@ -43,6 +43,7 @@ struct DeviceTables {
     double4 *pm = nullptr;     // [nsteps] pre/post-mix (ue_i, ue1_i, uo_i, 0):
                                //   P_{m+2i} = ue_i z_i + ue1_{i-1} z_{i-1},  P_{m+2i+1} = x uo_i z_i
     // per (order, latitude pair): recurrence start (scalar functions, and /sin for vectors)
+    int *tm = nullptr;         // [mmax+2] polar optimization: latitude pairs k < tm[slot] skipped
     int *i0 = nullptr;         // [(mmax+2) * nlat_2] first step with significant values
     double2 *z0 = nullptr;     // [(mmax+2) * nlat_2] (z_{i0-1}, z_{i0})
     // FFT: roots e^{2 pi i k/nphi}, k < nphi
@@ -66,6 +67,7 @@ struct shtns_info {
     int norm;
     int layout;                // SHT_THETA_CONTIGUOUS (native) or SHT_PHI_CONTIGUOUS
     double eps;                // polar threshold given to shtns_set_grid
+    std::vector<int> tm;       // [mmax+2] first latitude pair of each order slot (polar optimization)
     std::vector<double> wg;    // Gauss weights (raw, sum 2), nlat_2 values (north half)
     int nsteps_total = 0;      // coefficient slots over all orders (scalar and vector use)
     int device = -1;
@@ -89,6 +91,9 @@ struct shtns_info {
     // contraction precision chosen with cushtns_set_precision (-1: FP64 for double data,
     // FP32 for float data)
     int prec = -1;
+    // device copies of the host arrays of the host-pointer API (SH_to_spat, ...): slot i is
+    // the i-th array a call stages; kept and reused by later calls, grown when too small
+    std::vector<std::pair<void *, size_t>> hstage;
     // staging for the FP32-data entry points (FP64 copies of spectra / spatial fields)
     void *xscratch = nullptr;
     size_t xscratch_bytes = 0;
@@ -189,13 +194,14 @@ void launch_interleave(const double *xy, double2 *z, long z_dist, long n, int nf
 void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int nf, hipStream_t s);
 // Legendre transforms at one order (no FFT): table of P_lm at every latitude, sum over degrees,
 // weighted sum over latitudes
+// ktm: latitudes k < ktm and k >= nlat - ktm get zero tables (polar optimization of the order)
 void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
-                     double seed_sign, double *T, hipStream_t s);
+                     double seed_sign, int ktm, double *T, hipStream_t s);
 void launch_ml_synth(const double *T, int nlat, int nl, const double2 *Q, int real_only, double2 *V, hipStream_t s);
 void launch_ml_anal(const double *T, const double *w, int nlat, int nl, const double2 *V, int real_only, double scale,
                     double2 *Q, hipStream_t s);
 void launch_ml_vtable(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
-                      double seed_sign, double *Y, double *D, hipStream_t s);
+                      double seed_sign, int ktm, double *Y, double *D, hipStream_t s);
 void launch_ml_vsynth(const double *Y, const double *D, int nlat, int nl, int m, const double2 *S, const double2 *T,
                       double2 *Vt, double2 *Vp, hipStream_t s);
 void launch_ml_vanal(const double *Y, const double *D, const double *w, int nlat, int nl, int m, const double2 *Vt,

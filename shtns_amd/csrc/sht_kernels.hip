@@ -35,15 +35,23 @@ namespace shtns_amd {
 // functions exceed `thr` in magnitude, and (z_{i0-1}, z_{i0}) there.  The recurrence is
 // run with a power-of-two exponent carried beside the values (exact scaling), so orders
 // whose sin^m underflows double precision start correctly (SHTns does the same with its
-// SHT_SCALE_FACTOR, kernel_SH_to_spat.gen.c:278-348).
+// SHT_SCALE_FACTOR, kernel_SH_to_spat.gen.c:278-348).  Latitude pairs k < tm[slot] (the
+// polar optimization, computed on the host) never start: their rows of that order are
+// zero in synthesis and left out of the analysis sums.
 // ---------------------------------------------------------------------------------------
 __global__ void seed_kernel(const double *__restrict__ xk, const double *__restrict__ stk,
                             const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
-                            const double4 *__restrict__ pm, int lmax, int mmax, int mres, int nlat_2,
-                            double thr, int *__restrict__ i0_out, double2 *__restrict__ z0_out) {
+                            const double4 *__restrict__ pm, const int *__restrict__ tm, int lmax, int mmax,
+                            int mres, int nlat_2, double thr, int *__restrict__ i0_out,
+                            double2 *__restrict__ z0_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
     if (k >= nlat_2) return;
+    if (k < tm[slot]) {
+        i0_out[(size_t)slot * nlat_2 + k] = step_ofs[slot + 1] - step_ofs[slot];
+        z0_out[(size_t)slot * nlat_2 + k] = make_double2(0.0, 0.0);
+        return;
+    }
     const int m = (slot <= mmax) ? slot * mres : 1;
     const int nst = step_ofs[slot + 1] - step_ofs[slot];
     const double2 *abm = ab + step_ofs[slot];
@@ -108,8 +116,8 @@ __global__ void seed_kernel(const double *__restrict__ xk, const double *__restr
 
 void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s) {
     dim3 grid((nlat_2 + 63) / 64, mmax + 2);
-    hipLaunchKernelGGL(seed_kernel, grid, dim3(64), 0, s, t->x, t->st, t->step_ofs, t->ab, t->pm, lmax, mmax,
-                       mres, nlat_2, thr, t->i0, t->z0);
+    hipLaunchKernelGGL(seed_kernel, grid, dim3(64), 0, s, t->x, t->st, t->step_ofs, t->ab, t->pm, t->tm, lmax,
+                       mmax, mres, nlat_2, thr, t->i0, t->z0);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -312,13 +320,14 @@ void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int 
 // odd m with the Condon-Shortley phase), carrying a separate binary exponent so the
 // polar values underflow to zero only when they are below the double range.
 // ab[j] = (a_j, b_j): P_{m+j} = a_j x P_{m+j-1} + b_j P_{m+j-2} (ab[1].x = sqrt(2m+3)).
+// Latitudes within ktm of either pole (the polar optimization of this order) get zeros.
 __global__ void ml_table_kernel(const double *__restrict__ ct, const double *__restrict__ st, int nlat, int m, int nl,
-                                const double2 *__restrict__ ab, double log2_seed, double seed_sign,
+                                const double2 *__restrict__ ab, double log2_seed, double seed_sign, int ktm,
                                 double *__restrict__ T) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nlat) return;
     const double x = ct[k], s = st[k];
-    if (m > 0 && !(s > 0.0)) {
+    if ((m > 0 && !(s > 0.0)) || k < ktm || k >= nlat - ktm) {
         for (int j = 0; j < nl; ++j) T[(size_t)j * nlat + k] = 0.0;
         return;
     }
@@ -389,11 +398,18 @@ __global__ __launch_bounds__(256) void ml_anal_kernel(const double *__restrict__
 // For m = 0, dP_l0/dtheta = sqrt(l (l+1)) sin (P_l1/sin) from the order-1 recurrence (ab1,
 // seed a_11).  Every grid latitude has sin(theta) > 0.
 __global__ void ml_vtable_kernel(const double *__restrict__ ct, const double *__restrict__ st, int nlat, int m, int nl,
-                                 const double2 *__restrict__ ab, double log2_seed, double seed_sign,
+                                 const double2 *__restrict__ ab, double log2_seed, double seed_sign, int ktm,
                                  double *__restrict__ Y, double *__restrict__ D) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nlat) return;
     const double x = ct[k], s = st[k];
+    if (m >= 1 && (k < ktm || k >= nlat - ktm)) {   // polar optimization of this order
+        for (int o = 0; o < nl; ++o) {
+            Y[(size_t)o * nlat + k] = 0.0;
+            D[(size_t)o * nlat + k] = 0.0;
+        }
+        return;
+    }
     const int mr = (m >= 1) ? m : 1;     // order of the recurrence run here
     const double lg = log2_seed + (mr - 1) * log2(s);
     int e = (int)floor(lg);
@@ -487,9 +503,9 @@ __global__ __launch_bounds__(256) void ml_vanal_kernel(const double *__restrict_
 }
 
 void launch_ml_vtable(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
-                      double seed_sign, double *Y, double *D, hipStream_t s) {
+                      double seed_sign, int ktm, double *Y, double *D, hipStream_t s) {
     hipLaunchKernelGGL(ml_vtable_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, ct, st, nlat, m, nl, ab, log2_seed,
-                       seed_sign, Y, D);
+                       seed_sign, ktm, Y, D);
 }
 void launch_ml_vsynth(const double *Y, const double *D, int nlat, int nl, int m, const double2 *S, const double2 *T,
                       double2 *Vt, double2 *Vp, hipStream_t s) {
@@ -500,9 +516,9 @@ void launch_ml_vanal(const double *Y, const double *D, const double *w, int nlat
     hipLaunchKernelGGL(ml_vanal_kernel, dim3(nl), dim3(256), 0, s, Y, D, w, nlat, m, Vt, Vp, S, T);
 }
 void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
-                     double seed_sign, double *T, hipStream_t s) {
+                     double seed_sign, int ktm, double *T, hipStream_t s) {
     hipLaunchKernelGGL(ml_table_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, ct, st, nlat, m, nl, ab, log2_seed,
-                       seed_sign, T);
+                       seed_sign, ktm, T);
 }
 void launch_ml_synth(const double *T, int nlat, int nl, const double2 *Q, int real_only, double2 *V, hipStream_t s) {
     hipLaunchKernelGGL(ml_synth_kernel, dim3((nlat + 255) / 256), dim3(256), 0, s, T, nlat, nl, Q, real_only, V);

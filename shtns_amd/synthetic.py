@@ -55,3 +55,16 @@ def spectral_field(lmax, mmax, seed, mres=1, power=2.0):
 
 def spectral_batch(lmax, mmax, nfields, seed, mres=1, power=2.0):
     return np.stack([spectral_field(lmax, mmax, seed + f, mres, power) for f in range(nfields)])
+
+
+# Uneven-lmax workload: 256 scalar transforms with lmax uniform over these degrees
+# (SplitMix64, seed 4), each on a Gauss grid nlat = 3(lmax+1)/2, nphi = 2 nlat; field i
+# uses data seed 5 + i.
+UNEVEN_LMAX = (63, 127, 255, 511, 1023)
+UNEVEN_N, UNEVEN_SEED_L, UNEVEN_SEED_DATA = 256, 4, 5
+
+
+def uneven_lmaxes(n=UNEVEN_N, seed=UNEVEN_SEED_L):
+    """lmax of the n transforms of the uneven workload."""
+    u = splitmix64(seed, n)
+    return [UNEVEN_LMAX[int(x % len(UNEVEN_LMAX))] for x in u]

@@ -1,0 +1,180 @@
+"""GPU: the benchmark configurations at their full sizes, against the oracle.
+
+* lmax=1023 on 1536 x 3072 (16-field batch): analysis of one field against the oracle's
+  analysis (synthesis is checked in test_gpu_parity.py);
+* vector transforms, lmax=511 on 768 x 1536, 32 (S,T) pairs (S, T ~ N(0,(l+1)^-3), seeds
+  2+f and 34+f): every pair against the oracle's vector synthesis and analysis (FP64,
+  rel-L2 <= 1e-12 per field), and the FP32-split contraction's error per field, printed and
+  bounded;
+* the uneven batch: 256 scalar transforms with lmax drawn from {63,...,1023} (Gauss
+  nlat = 3(lmax+1)/2, nphi = 2 nlat) run from one HIP graph per direction: two seeded
+  fields of every lmax against the oracle (synthesis, and analysis of unfiltered data), a
+  round trip for every field, and the FP32-split batch's error against FP64;
+* one lmax=4095 field on 6144 x 12288: synthesis against the oracle on 8 latitudes (every
+  order), analysis of unfiltered data against the oracle on 6 orders (every degree), and a
+  round trip.  The sampled oracle values are pinned against its full transforms in
+  tests/test_sampled_oracle.py.
+
+All transforms use polar_opt = 0 (the benchmark's setting); the polar threshold is covered
+by test_gpu_polar.py."""
+import numpy as np
+import pytest
+import torch
+
+import shtns_amd
+from shtns_amd.batch import UnevenBatch
+from shtns_amd.synthetic import (UNEVEN_LMAX, UNEVEN_SEED_DATA, spectral_batch, spectral_field,
+                                 uneven_lmaxes)
+from oracle.oracle import Oracle
+from sampled_oracle import anal_orders, spat_rows, synth_rows
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((np.asarray(a) - b).ravel()) / np.linalg.norm(np.asarray(b).ravel()))
+
+
+def make(lmax, nlat, nphi):
+    s = shtns_amd.sht(lmax)
+    s.set_grid(nlat, nphi, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss, polar_opt=0.0)
+    return s
+
+
+def test_lmax1023_analysis_against_oracle(device):
+    L, nlat, nphi, nb = 1023, 1536, 3072, 16
+    s = make(L, nlat, nphi)
+    o = Oracle(L, L, 1, nlat, nphi)
+    rng = np.random.default_rng(40)
+    # a batch of 16: field 5 synthesized by the oracle from the seed-6 coefficients, field 11
+    # unfiltered noise, the others the GPU's own synthesis of their coefficients
+    q = spectral_batch(L, L, nb, seed=1)
+    v = s.synth(torch.from_numpy(q).to(device))
+    v5 = o.synth(q[5])
+    w11 = rng.standard_normal((nphi, nlat))
+    v[5] = torch.from_numpy(v5).to(device)
+    v[11] = torch.from_numpy(w11).to(device)
+    qa = s.analys(v).cpu().numpy()
+    assert rel_l2(qa[5], o.analys(v5)) <= TOL
+    assert rel_l2(qa[11], o.analys(w11)) <= TOL
+    assert rel_l2(qa[5], q[5]) <= TOL
+    for f in (0, 15):
+        assert rel_l2(qa[f], q[f]) <= TOL
+
+
+def test_vector_lmax511_32_pairs(device):
+    L, nlat, nphi, nb = 511, 768, 1536, 32
+    s = make(L, nlat, nphi)
+    o = Oracle(L, L, 1, nlat, nphi)
+    S = spectral_batch(L, L, nb, seed=2, power=3.0)
+    T = spectral_batch(L, L, nb, seed=2 + nb, power=3.0)
+    S[:, 0] = 0
+    T[:, 0] = 0
+    St, Tt = torch.from_numpy(S).to(device), torch.from_numpy(T).to(device)
+    vt, vp = (x.cpu().numpy() for x in s.synth(St, Tt))
+    refs = [o.synth_sphtor(S[f], T[f]) for f in range(nb)]
+    Vt = torch.from_numpy(np.stack([r[0] for r in refs])).to(device)
+    Vp = torch.from_numpy(np.stack([r[1] for r in refs])).to(device)
+    s2, t2 = (x.cpu().numpy() for x in s.analys(Vt, Vp))
+    worst = 0.0
+    for f in range(nb):
+        sa, ta = o.analys_sphtor(*refs[f])
+        e = max(rel_l2(vt[f], refs[f][0]), rel_l2(vp[f], refs[f][1]), rel_l2(s2[f], sa), rel_l2(t2[f], ta),
+                rel_l2(s2[f], S[f]), rel_l2(t2[f], T[f]))
+        worst = max(worst, e)
+        assert e <= TOL, (f, e)
+    print(f"vector lmax=511, 32 pairs, FP64: worst per-field rel-L2 vs oracle {worst:.2e}")
+    # FP32-split contraction: error of every field against the FP64 oracle
+    s.set_precision(shtns_amd.SHT_PREC_SPLIT)
+    try:
+        vt2, vp2 = (x.cpu().numpy() for x in s.synth(St, Tt))
+        s3, t3 = (x.cpu().numpy() for x in s.analys(Vt, Vp))
+    finally:
+        s.set_precision(shtns_amd.SHT_PREC_AUTO)
+    es = [max(rel_l2(vt2[f], refs[f][0]), rel_l2(vp2[f], refs[f][1])) for f in range(nb)]
+    ea = [max(rel_l2(s3[f], S[f]), rel_l2(t3[f], T[f])) for f in range(nb)]
+    print("vector lmax=511 FP32-split per-field error vs FP64 oracle: synthesis max %.2e median %.2e, "
+          "analysis max %.2e median %.2e" % (max(es), float(np.median(es)), max(ea), float(np.median(ea))))
+    assert max(es) <= 1e-5 and max(ea) <= 1e-5
+
+
+def _uneven_grids():
+    cfg = {}
+    for L in UNEVEN_LMAX:
+        nlat = 3 * (L + 1) // 2
+        cfg[L] = (make(L, nlat, 2 * nlat), Oracle(L, L, 1, nlat, 2 * nlat))
+    return cfg
+
+
+def test_uneven_batch_256(device):
+    lmaxes = uneven_lmaxes()
+    assert len(lmaxes) == 256 and set(lmaxes) == set(UNEVEN_LMAX)
+    cfg = _uneven_grids()
+    qs = [spectral_field(L, L, seed=UNEVEN_SEED_DATA + i) for i, L in enumerate(lmaxes)]
+    sample = []
+    for L in UNEVEN_LMAX:
+        sample += [i for i, x in enumerate(lmaxes) if x == L][:2]
+    rng = np.random.default_rng(41)
+    results = {}
+    for name, prec in (("fp64", shtns_amd.SHT_PREC_F64), ("split", shtns_amd.SHT_PREC_SPLIT)):
+        b = UnevenBatch([cfg[L][0] for L in lmaxes], prec=prec)
+        try:
+            for i in range(256):
+                b.arrays(i)[0].copy_(torch.from_numpy(qs[i]))
+            b.synth()
+            torch.cuda.synchronize()
+            v = [b.arrays(i)[1].cpu().numpy() for i in range(256)]
+            # unfiltered data in the second sampled field of every lmax
+            noise = {}
+            for j, i in enumerate(sample):
+                if j % 2:
+                    noise[i] = rng.standard_normal(v[i].shape)
+                    b.arrays(i)[1].copy_(torch.from_numpy(noise[i]))
+            b.analys()
+            torch.cuda.synchronize()
+            qa = [b.arrays(i)[0].cpu().numpy() for i in range(256)]
+        finally:
+            b.close()
+        results[name] = (v, qa)
+        if name != "fp64":
+            continue
+        for i in sample:
+            o = cfg[lmaxes[i]][1]
+            assert rel_l2(v[i], o.synth(qs[i])) <= TOL, (i, lmaxes[i])
+            if i in noise:
+                assert rel_l2(qa[i], o.analys(noise[i])) <= TOL, (i, lmaxes[i])
+        for i in range(256):
+            if i not in noise:
+                assert rel_l2(qa[i], qs[i]) <= TOL, (i, lmaxes[i])
+    # FP32-split: error of the whole batch against the FP64 batch
+    v64, q64 = results["fp64"]
+    vs, qsp = results["split"]
+    es = max(rel_l2(vs[i], v64[i]) for i in range(256))
+    ea = max(rel_l2(qsp[i], q64[i]) for i in range(256))
+    print(f"uneven batch FP32-split vs FP64: worst field synthesis {es:.2e}, analysis {ea:.2e}")
+    assert es <= 1e-5 and ea <= 1e-5
+
+
+def test_lmax4095_field(device):
+    L, nlat, nphi = 4095, 6144, 12288
+    s = make(L, nlat, nphi)
+    o = Oracle(L, L, 1, nlat, nphi)
+    q = spectral_field(L, L, seed=UNEVEN_SEED_DATA + 256)
+    qt = torch.from_numpy(q).to(device)
+    vt = s.synth(qt)
+    lats = [0, 1, 700, 2047, 3071, 3072, 5000, nlat - 1]
+    G = synth_rows(o, q, lats)
+    R = spat_rows(vt[:, lats].cpu().numpy(), list(range(len(lats))))[:L + 1]
+    R[0] = R[0].real
+    e_s = rel_l2(R, G)
+    # round trip, and the analysis of unfiltered data against the oracle on sampled orders
+    qr = s.analys(vt).cpu().numpy()
+    e_rt = rel_l2(qr, q)
+    w = torch.randn((nphi, nlat), dtype=torch.float64, device=device, generator=torch.Generator(device).manual_seed(42))
+    qw = s.analys(w).cpu().numpy()
+    ims = [0, 1, 100, 1500, 3000, 4095]
+    ref = anal_orders(o, w.cpu().numpy(), ims)
+    e_a = max(rel_l2(qw[o.idx(m, m):o.idx(m, m) + L + 1 - m], ref[m]) for m in ims)
+    print(f"lmax=4095: synthesis on 8 latitudes {e_s:.2e}, analysis on 6 orders {e_a:.2e}, round trip {e_rt:.2e}")
+    assert e_s <= TOL and e_a <= TOL and e_rt <= TOL

@@ -263,3 +263,58 @@ def test_checks_reject_planted_mistakes():
     }
     for name, qb in bad.items():
         assert rel(o.synth(qb), ref) > 1e-3, name
+
+
+# ---- polar optimization (sht_init.c:522-557; applied as kernel_SH_to_spat.gen.c:235-257,
+#      kernel_spat_to_SH.gen.c:198-201) ----
+def _brute_tm(o, eps):
+    """tm[im] from its definition: first latitude (from the pole) where max_l |Y_l^m| >= eps,
+    never below tm[im-1], rounded down to even -- computed with scipy's harmonics."""
+    L, nl2 = o.lmax, o.nlat // 2
+    th = np.arccos(o.cos_theta[:nl2])
+    tm, it = [0], 0
+    for im in range(1, o.mmax + 1):
+        m = im * o.mres
+        ls = np.arange(m, L + 1)
+        while it < nl2 and np.abs(np.real(sph_harm_y(ls, m, th[it], 0.0))).max() < eps:
+            it += 1
+        tm.append(it)
+    return np.array([(t // 2) * 2 for t in tm])
+
+
+@pytest.mark.parametrize("eps", [1e-10, 1e-4])
+def test_polar_tm_vs_definition(eps):
+    o = Oracle(63, 63, 1, 96, 128)
+    o.set_polar_opt(eps)
+    ref = _brute_tm(o, eps)
+    assert np.array_equal(o.polar_tm(), ref)
+    assert ref.max() > 0   # the threshold skips something at these sizes
+
+
+def test_polar_skip_semantics():
+    """With eps > 0, order m's rows at latitudes k < tm[m] (both hemispheres) are zero in
+    synthesis and left out of the analysis; every other value equals the eps = 0 result."""
+    L, nlat, nphi, eps = 63, 96, 128, 1e-4
+    o0, oe = Oracle(L, L, 1, nlat, nphi), Oracle(L, L, 1, nlat, nphi)
+    oe.set_polar_opt(eps)
+    tm = oe.polar_tm()
+    rng = np.random.default_rng(5)
+    for m in (12, 30, 55):
+        assert tm[m] > 0
+        q = np.zeros(o0.nlm, dtype=complex)
+        i0 = o0.idx(m, m)
+        q[i0:i0 + L + 1 - m] = rng.standard_normal(L + 1 - m) + 1j * rng.standard_normal(L + 1 - m)
+        v0, ve = o0.synth(q), oe.synth(q)
+        rows = np.r_[0:tm[m], nlat - tm[m]:nlat]
+        keep = np.setdiff1d(np.arange(nlat), rows)
+        assert np.abs(ve[:, rows]).max() == 0.0
+        assert rel(ve[:, keep], v0[:, keep]) <= 1e-13
+    # analysis: order m of A_eps(V) equals order m of A_0(V with those rows zeroed)
+    v = rng.standard_normal((nphi, nlat))
+    qe = oe.analys(v)
+    for m in (0, 12, 30, 55):
+        vm = v.copy()
+        vm[:, np.r_[0:tm[m], nlat - tm[m]:nlat]] = 0.0
+        qm = o0.analys(vm)
+        sl = slice(o0.idx(m, m), o0.idx(m, m) + L + 1 - m)
+        assert rel(qe[sl], qm[sl]) <= 1e-13
