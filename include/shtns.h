@@ -185,6 +185,28 @@ void SHqst_to_spat_l(shtns_cfg, cplx *Qlm, cplx *Slm, cplx *Tlm, double *Vr, dou
 
 /* rotation about the z axis by alpha, on the host (shtns.h:188-191); Rlm may be Qlm */
 void SH_Zrotate(shtns_cfg, cplx *Qlm, double alpha, cplx *Rlm);
+/* rotations about y by alpha, about y and x by 90 degrees (shtns.h:193-198); need mres = 1,
+   mmax = lmax.  A rotation by +a maps the field f to f(R^-1 x) (active, right-handed). */
+void SH_Yrotate(shtns_cfg, cplx *Qlm, double alpha, cplx *Rlm);
+void SH_Yrotate90(shtns_cfg, cplx *Qlm, cplx *Rlm);
+void SH_Xrotate90(shtns_cfg, cplx *Qlm, cplx *Rlm);
+
+/* rotation objects (shtns.h:200-218), orthonormal harmonics (norm may add SHT_NO_CS_PHASE,
+   SHT_REAL_NORM).  set_angles_ZYZ(a, b, g): rotate by a about z, then b about y, then g
+   about z, i.e. f'(x) = f(R^-1 x) with R = Rz(g) Ry(b) Rz(a); set_angles_ZXZ likewise with
+   x; set_angle_axis: by theta about the axis (Vx, Vy, Vz).  wigner_d_matrix fills
+   mx[(m'+l)(2l+1) + (m+l)] = d^l_{m'm}(b) and returns 2l+1 (0 on error).  apply_real takes
+   nlm_calc(lmax, mmax, 1) coefficients of a real field, apply_cplx nlm_cplx coefficients of
+   a complex field (LM_cplx layout); Rlm may be the input array. */
+typedef struct shtns_rot_ *shtns_rot;
+shtns_rot shtns_rotation_create(const int lmax, const int mmax, int norm);
+void shtns_rotation_destroy(shtns_rot r);
+void shtns_rotation_set_angles_ZYZ(shtns_rot r, double alpha, double beta, double gamma);
+void shtns_rotation_set_angles_ZXZ(shtns_rot r, double alpha, double beta, double gamma);
+void shtns_rotation_set_angle_axis(shtns_rot r, double theta, double Vx, double Vy, double Vz);
+int shtns_rotation_wigner_d_matrix(shtns_rot r, const int l, double *mx);
+void shtns_rotation_apply_cplx(shtns_rot r, cplx *Zlm, cplx *Rlm);
+void shtns_rotation_apply_real(shtns_rot r, cplx *Qlm, cplx *Rlm);
 
 /* point and latitude evaluation, on the host; no grid needed (shtns.h:352-362) */
 double SH_to_point(shtns_cfg, cplx *Qlm, double cost, double phi);

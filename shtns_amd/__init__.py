@@ -34,7 +34,7 @@ SHT_ALLOW_PADDING = 256 * 256
 # contraction precision of the GPU Legendre step (shtns_hip.h)
 SHT_PREC_AUTO, SHT_PREC_F64, SHT_PREC_F32, SHT_PREC_SPLIT = -1, 0, 1, 2
 
-__all__ = ["sht", "nlm_calc", "build_info"]
+__all__ = ["sht", "rotation", "nlm_calc", "build_info"]
 
 
 def nlm_calc(lmax, mmax, mres):
@@ -202,6 +202,27 @@ class sht:
         r = np.empty_like(q)
         _lib.load().SH_Zrotate(self._cfg, _ptr(q), float(alpha), _ptr(r))
         return r
+
+    def _rotate(self, fn, Qlm, *args):
+        if self.mres != 1 or self.mmax != self.lmax:
+            raise RuntimeError("truncature makes rotation not closed.")
+        self._host_spec(1, Qlm)
+        r = np.empty_like(Qlm)
+        fn(self._cfg, _ptr(Qlm), *args, _ptr(r))
+        return r
+
+    def Yrotate(self, Qlm, alpha):
+        """Rotate the field about the y axis by alpha (shtns_numpy.i:617-627); mres = 1 and
+        mmax = lmax are required."""
+        return self._rotate(_lib.load().SH_Yrotate, Qlm, float(alpha))
+
+    def Yrotate90(self, Qlm):
+        """Rotate the field by 90 degrees about the y axis (shtns_numpy.i:628-638)."""
+        return self._rotate(_lib.load().SH_Yrotate90, Qlm)
+
+    def Xrotate90(self, Qlm):
+        """Rotate the field by 90 degrees about the x axis (shtns_numpy.i:639-649)."""
+        return self._rotate(_lib.load().SH_Xrotate90, Qlm)
 
     def spat_to_SH_m(self, Vr, Qlm, im):
         """Legendre analysis at order m = im*mres, no FFT: Vr complex[nlat] -> Qlm complex[lmax+1-m]
@@ -689,3 +710,70 @@ class sht:
                                                     self.nlm, self.nspat)
             _lib.check_error("analys_adjoint")
         return out
+
+
+class rotation:
+    """Rotation of spherical harmonic expansions (SHTns's ``shtns.rotation``,
+    shtns_numpy.i:715-800).  A rotation maps the field f to f(R^-1 x); set_angles_ZYZ(a, b, g)
+    rotates by a about z, then b about y, then g about z."""
+
+    def __init__(self, lmax, mmax=-1, norm=0):
+        if lmax < 2:
+            raise ValueError("lmax < 2 not allowed")
+        if mmax < 0:
+            mmax = lmax
+        if mmax > lmax:
+            raise ValueError("lmax < mmax invalid")
+        self.lmax, self.mmax = int(lmax), int(mmax)
+        self._r = _lib.load().shtns_rotation_create(self.lmax, self.mmax, int(norm))
+        if not self._r:
+            _lib.check_error("rotation")
+        self.nlm = nlm_calc(self.lmax, self.mmax, 1)
+        self.nlm_cplx = int(_lib.load().nlm_cplx_calc(self.lmax, self.mmax, 1))
+
+    def __del__(self):
+        r = getattr(self, "_r", None)
+        if r:
+            try:
+                _lib.load().shtns_rotation_destroy(r)
+            except Exception:
+                pass
+            self._r = None
+
+    def set_angles_ZYZ(self, alpha, beta, gamma):
+        if abs(beta) > np.pi:
+            raise ValueError("beta must be between -pi and pi")
+        _lib.load().shtns_rotation_set_angles_ZYZ(self._r, float(alpha), float(beta), float(gamma))
+
+    def set_angles_ZXZ(self, alpha, beta, gamma):
+        if abs(beta) > np.pi:
+            raise ValueError("beta must be between -pi and pi")
+        _lib.load().shtns_rotation_set_angles_ZXZ(self._r, float(alpha), float(beta), float(gamma))
+
+    def set_angle_axis(self, theta, Vx, Vy, Vz):
+        _lib.load().shtns_rotation_set_angle_axis(self._r, float(theta), float(Vx), float(Vy), float(Vz))
+        _lib.check_error("set_angle_axis")
+
+    def wigner_d_matrix(self, l):
+        """d^l_{m'm}(beta) as a (2l+1, 2l+1) array, rows m' = -l..l, columns m = -l..l."""
+        if l < 0 or l > self.lmax:
+            raise ValueError("l must be between 0 and lmax")
+        mx = np.empty((2 * l + 1, 2 * l + 1))
+        _lib.load().shtns_rotation_wigner_d_matrix(self._r, int(l), _ptr(mx))
+        return mx
+
+    def _apply(self, fn, Qlm, n):
+        q = np.ascontiguousarray(Qlm, dtype=np.complex128)
+        if q.size != n:
+            raise RuntimeError("spectral array has wrong size")
+        r = np.empty(n, dtype=np.complex128)
+        fn(self._r, _ptr(q), _ptr(r))
+        return r
+
+    def apply_real(self, Qlm):
+        """Rotated coefficients of a real field (nlm_calc(lmax, mmax, 1) values, LM order)."""
+        return self._apply(_lib.load().shtns_rotation_apply_real, Qlm, self.nlm)
+
+    def apply_cplx(self, Zlm):
+        """Rotated coefficients of a complex field (LM_cplx layout)."""
+        return self._apply(_lib.load().shtns_rotation_apply_cplx, Zlm, self.nlm_cplx)

@@ -24,6 +24,9 @@ EXPORTED_SYMBOLS = [
     "SHqst_to_spat_ml", "spat_to_SHqst_ml",
     "SH_to_point_cplx", "SH_to_grad_point", "SHqst_to_point", "SH_to_lat", "SHqst_to_lat", "SH_Zrotate",
     "legendre_sphPlm_deriv_array", "mul_ct_matrix", "st_dt_matrix", "SH_mul_mx",
+    "SH_Yrotate", "SH_Yrotate90", "SH_Xrotate90", "shtns_rotation_create", "shtns_rotation_destroy",
+    "shtns_rotation_set_angles_ZYZ", "shtns_rotation_set_angles_ZXZ", "shtns_rotation_set_angle_axis",
+    "shtns_rotation_wigner_d_matrix", "shtns_rotation_apply_cplx", "shtns_rotation_apply_real",
     "shtns_last_error", "shtns_last_error_string",
     # shtns_hip.h
     "cu_spat_to_SH", "cu_SH_to_spat", "cu_spat_to_SHsphtor", "cu_SHsphtor_to_spat",
@@ -117,6 +120,17 @@ def load():
         "SHqst_to_point": (None, [CFG, P, P, P, D, D, P, P, P]),
         "SH_to_lat": (None, [CFG, P, D, P, I, I, I]),
         "SH_Zrotate": (None, [CFG, P, D, P]),
+        "SH_Yrotate": (None, [CFG, P, D, P]),
+        "SH_Yrotate90": (None, [CFG, P, P]),
+        "SH_Xrotate90": (None, [CFG, P, P]),
+        "shtns_rotation_create": (P, [I, I, I]),
+        "shtns_rotation_destroy": (None, [P]),
+        "shtns_rotation_set_angles_ZYZ": (None, [P, D, D, D]),
+        "shtns_rotation_set_angles_ZXZ": (None, [P, D, D, D]),
+        "shtns_rotation_set_angle_axis": (None, [P, D, D, D, D]),
+        "shtns_rotation_wigner_d_matrix": (I, [P, I, P]),
+        "shtns_rotation_apply_cplx": (None, [P, P, P]),
+        "shtns_rotation_apply_real": (None, [P, P, P]),
         "legendre_sphPlm_deriv_array": (I, [CFG, I, I, D, D, P, P]),
         "mul_ct_matrix": (None, [CFG, P]),
         "st_dt_matrix": (None, [CFG, P]),

@@ -116,6 +116,12 @@ def test_uneven_batch_256(device):
     for L in UNEVEN_LMAX:
         sample += [i for i, x in enumerate(lmaxes) if x == L][:2]
     rng = np.random.default_rng(41)
+    # unfiltered data replaces the synthesis of the second sampled field of every lmax
+    noise = {}
+    for j, i in enumerate(sample):
+        if j % 2:
+            nl = 3 * (lmaxes[i] + 1) // 2
+            noise[i] = rng.standard_normal((2 * nl, nl))
     results = {}
     for name, prec in (("fp64", shtns_amd.SHT_PREC_F64), ("split", shtns_amd.SHT_PREC_SPLIT)):
         b = UnevenBatch([cfg[L][0] for L in lmaxes], prec=prec)
@@ -125,12 +131,9 @@ def test_uneven_batch_256(device):
             b.synth()
             torch.cuda.synchronize()
             v = [b.arrays(i)[1].cpu().numpy() for i in range(256)]
-            # unfiltered data in the second sampled field of every lmax
-            noise = {}
-            for j, i in enumerate(sample):
-                if j % 2:
-                    noise[i] = rng.standard_normal(v[i].shape)
-                    b.arrays(i)[1].copy_(torch.from_numpy(noise[i]))
+            for i, w in noise.items():
+                assert w.shape == v[i].shape
+                b.arrays(i)[1].copy_(torch.from_numpy(w))
             b.analys()
             torch.cuda.synchronize()
             qa = [b.arrays(i)[0].cpu().numpy() for i in range(256)]
