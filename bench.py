@@ -218,7 +218,7 @@ def main():
     if not args.no_prec:
         v64 = v.clone()
         q64 = q2.clone()
-        for name, code in (("fp32_split", 2), ("fp32", 1)):
+        for name, code in (("int8_sliced", 3), ("fp32_split", 2), ("fp32", 1)):
             el, kp = timed_loop(code)
             ev = float(torch.linalg.norm((v - v64).flatten()) / torch.linalg.norm(v64.flatten()))
             eq = float(torch.linalg.norm((q2 - q64).flatten()) / torch.linalg.norm(q64.flatten()))

@@ -92,11 +92,16 @@ void cu_spat_to_SH_float_batch(shtns_cfg, float *Vr, cplx_f *Qlm, int ltr, int n
  *                   float data);
  *   SHT_PREC_SPLIT  both operands as FP32 hi+lo pairs, three FP32 matrix-core products
  *                   (hi*hi + lo*hi + hi*lo), same accumulation.
+ *   SHT_PREC_SLICED both operands as five signed 8-bit digits with power-of-two scales per
+ *                   latitude / column and per 128 recurrence steps, exact int32 sums of the
+ *                   19 leading digit products on the int8 matrix cores (scalar synthesis of
+ *                   double data; other transforms run the FP64 contraction in this mode).
  * SHT_PREC_AUTO restores the defaults.  Returns 0, or -1 for an unknown value. */
 #define SHT_PREC_AUTO (-1)
 #define SHT_PREC_F64 0
 #define SHT_PREC_F32 1
 #define SHT_PREC_SPLIT 2
+#define SHT_PREC_SLICED 3
 int cushtns_set_precision(shtns_cfg, int prec);
 
 /* Autotuner (additive, no SHTns counterpart).  Times the candidate kernel shapes of this

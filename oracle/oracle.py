@@ -37,6 +37,8 @@ def _load():
         D = ctypes.c_double
         lib.orc_create.restype = P
         lib.orc_create.argtypes = [I, I, I, I, I]
+        lib.orc_create_grid.restype = P
+        lib.orc_create_grid.argtypes = [I, I, I, I, I, I]
         lib.orc_destroy.argtypes = [P]
         lib.orc_nlm.restype = ctypes.c_long
         lib.orc_nlm.argtypes = [P]
@@ -72,7 +74,9 @@ def fft(x, sign=-1):
 class Oracle:
     """One Gauss-grid configuration of the oracle."""
 
-    def __init__(self, lmax, mmax=None, mres=1, nlat=None, nphi=None):
+    GAUSS, REGULAR, REGULAR_POLES = 0, 1, 2
+
+    def __init__(self, lmax, mmax=None, mres=1, nlat=None, nphi=None, grid=0):
         if mmax is None:
             mmax = lmax // mres
         if nlat is None:
@@ -81,7 +85,7 @@ class Oracle:
             nphi = 2 * mmax + 2
         self.lmax, self.mmax, self.mres, self.nlat, self.nphi = lmax, mmax, mres, nlat, nphi
         lib = _load()
-        self._h = lib.orc_create(lmax, mmax, mres, nlat, nphi)
+        self._h = lib.orc_create_grid(lmax, mmax, mres, nlat, nphi, grid)
         if not self._h:
             raise ValueError("invalid sizes for oracle config")
         self.nlm = int(lib.orc_nlm(self._h))

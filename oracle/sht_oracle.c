@@ -45,6 +45,7 @@ struct orc_cfg {
     int lmax, mmax, mres, nlat, nlat_2, nphi;
     long nlm;
     double *ct, *st, *w;      /* nlat each */
+    double *wp;               /* pair weights: w, halved at the middle row of an odd grid */
     int mtab_n;               /* tables for m = 0..mtab_n-1 (only needed ones filled) */
     double **seed;            /* seed[m] = a_mm (1 value) */
     double **ab;              /* ab[m][2*(l-m)] = b_l, [2*(l-m)+1] = a_l ; l = m..lmax+1 */
@@ -88,6 +89,18 @@ static void gauss_nodes_(double *x, double *st, double *w, int n) {
         st[i] = sqrt(s2);
         st[n - 1 - i] = st[i];
     }
+    if (n & 1) {   /* odd n: the middle root x = 0, weight 2 / (n P_{n-1}(0))^2 */
+        double p1 = 0.0, p2 = 1.0;   /* P_1(0), P_0(0) */
+        for (long l = 2; l < n; ++l) {
+            double p3 = p2;
+            p2 = p1;
+            p1 = ((2 * l - 1) * 0.0 * p2 - (l - 1) * p3) / l;
+        }
+        /* p1 = P_{n-1}(0) */
+        x[m] = 0.0;
+        st[m] = 1.0;
+        w[m] = 2.0 / ((n * p1) * (n * p1));
+    }
 }
 
 /* ---- recurrence coefficients for one order m (sht_legendre.c:584-613, 646-657) ---- */
@@ -125,14 +138,20 @@ static void fill_m_table_(orc_cfg *c, int m) {
 
 /* ---- one column y_l(x) = Y_l^m(x), l=m..ltop, with an exponent carried in powers of two
  *      while the values are below the double range (extended-range start). ---- */
+static void leg_column_pow_(const orc_cfg *c, int m, double x, double sint, int ltop, double *y, int sub);
 static void leg_column_m_(const orc_cfg *c, int m, double x, double sint, int ltop, double *y) {
+    leg_column_pow_(c, m, x, sint, ltop, y, 0);
+}
+/* Y_l^m(x) / sint^sub: the same recurrence from the seed a_mm sint^(m-sub) (sub <= m), finite
+   at the poles (the vector transforms use sub = 1 for their functions Y/sin(theta)) */
+static void leg_column_pow_(const orc_cfg *c, int m, double x, double sint, int ltop, double *y, int sub) {
     const double *ab = c->ab[m];
-    /* seed a_mm * sint^m as mantissa * 2^E by binary powering with renormalisation */
+    /* seed a_mm * sint^(m-sub) as mantissa * 2^E by binary powering with renormalisation */
     int E = 0, es;
     double mant = frexp(c->seed[m][0], &es);
     E += es;
     {
-        int n = m;
+        int n = m - sub;
         double b = frexp(sint, &es);
         int eb = es;
         while (n) {
@@ -182,18 +201,51 @@ void orc_legendre(const orc_cfg *c, int im, double x, double sint, int ltop, dou
     leg_column_m_(c, im * c->mres, x, sint, ltop, yl);
 }
 
+/* ---- regular grids (sht_init.c:580-597): Fejer's first rule without the poles (nodes
+ *      (k+1/2) pi/n) and Clenshaw-Curtis with the poles (nodes k pi/(n-1)), weights from their
+ *      closed forms (Waldvogel, BIT 46 (2006)):
+ *        Fejer:  w_k = (2/n) [1 - 2 sum_{j=1}^{n/2} cos(2 j t_k) / (4 j^2 - 1)]
+ *        CC:     w_k = (c_k/N) [1 - sum_{j=1}^{N/2} b_j cos(2 j t_k) / (4 j^2 - 1)],  N = n-1,
+ *                c_0 = c_N = 1, c_k = 2 otherwise; b_j = 1 for j = N/2, 2 otherwise ---- */
+static void regular_nodes_(double *x, double *st, double *w, int n, int poles) {
+    const int N = poles ? n - 1 : n;
+    for (int k = 0; k < n; ++k) {
+        const double t = poles ? M_PI * k / N : M_PI * (k + 0.5) / n;
+        double sum = 0.0;
+        for (int j = 1; j <= N / 2; ++j) {
+            const double b = poles ? ((2 * j == N) ? 1.0 : 2.0) : 2.0;
+            sum += b * cos(2.0 * j * t) / (4.0 * j * j - 1.0);
+        }
+        const double ck = poles ? ((k == 0 || k == N) ? 1.0 : 2.0) : 2.0;
+        w[k] = ck / N * (1.0 - sum);
+        x[k] = cos(t);
+        st[k] = sin(t);
+    }
+    if (poles) { st[0] = 0.0; st[n - 1] = 0.0; x[0] = 1.0; x[n - 1] = -1.0; }
+    if (n & 1) { x[n / 2] = 0.0; st[n / 2] = 1.0; }
+}
+
 /* ---- create / destroy ---- */
 orc_cfg *orc_create(int lmax, int mmax, int mres, int nlat, int nphi) {
+    return orc_create_grid(lmax, mmax, mres, nlat, nphi, 0);
+}
+
+orc_cfg *orc_create_grid(int lmax, int mmax, int mres, int nlat, int nphi, int grid) {
     if (lmax < 1 || mres < 1 || mmax < 0 || mmax * mres > lmax) return NULL;
-    if (nlat <= lmax || (nlat & 1) || nphi <= 2 * mmax) return NULL;
+    if (nlat <= lmax || nphi <= 2 * mmax || grid < 0 || grid > 2 || nlat < 2) return NULL;
     orc_cfg *c = (orc_cfg *)calloc(1, sizeof(orc_cfg));
     c->lmax = lmax; c->mmax = mmax; c->mres = mres;
-    c->nlat = nlat; c->nlat_2 = nlat / 2; c->nphi = nphi;
+    c->nlat = nlat; c->nlat_2 = (nlat + 1) / 2; c->nphi = nphi;
     c->nlm = (long)(mmax + 1) * (lmax + 1) - (long)mres * mmax * (mmax + 1) / 2;
-    c->ct = (double *)malloc(3 * sizeof(double) * nlat);
+    c->ct = (double *)malloc(4 * sizeof(double) * nlat);
     c->st = c->ct + nlat;
     c->w = c->st + nlat;
-    gauss_nodes_(c->ct, c->st, c->w, nlat);
+    c->wp = c->w + nlat;
+    if (grid == 0) gauss_nodes_(c->ct, c->st, c->w, nlat);
+    else regular_nodes_(c->ct, c->st, c->w, nlat, grid == 2);
+    /* weight of latitude pair k in the analysis sums (N + S) w: the middle row of an odd grid
+       is its own mirror, so its weight is halved there (sht_init.c:598) */
+    for (int k = 0; k < nlat; ++k) c->wp[k] = (2 * k + 1 == nlat) ? 0.5 * c->w[k] : c->w[k];
     c->mtab_n = mmax * mres + 2;
     c->seed = (double **)calloc(c->mtab_n, sizeof(double *));
     c->ab = (double **)calloc(c->mtab_n, sizeof(double *));
@@ -379,7 +431,7 @@ void orc_spat_to_SH(const orc_cfg *c, const double *Vr, double *Q, int ltr) {
         double *acc = (double *)calloc(2 * (size_t)(ltr - m + 1), sizeof(double));
         for (int k = c->tm[im]; k < nl2; ++k) {
             leg_column_m_(c, m, c->ct[k], c->st[k], ltr, y);
-            const double wk = c->w[k] * wfac;
+            const double wk = c->wp[k] * wfac;
             const double *Fn = F + 2 * ((size_t)k * mm1 + im);
             const double *Fs = F + 2 * ((size_t)(nlat - 1 - k) * mm1 + im);
             double er = (Fn[0] + Fs[0]) * wk, ei = (Fn[1] + Fs[1]) * wk;
@@ -454,8 +506,9 @@ void orc_SHsphtor_to_spat(const orc_cfg *c, const double *S, const double *T, do
         }
         const int ltop = (m == 0) ? ltr : ltr + 1;
         for (int k = c->tm[im]; k < nl2; ++k) {
-            leg_column_m_(c, mcol, c->ct[k], c->st[k], ltop, y);
-            const double s1 = (m == 0) ? 1.0 : 1.0 / c->st[k];
+            /* Y/sin(theta) for m >= 1 straight from the recurrence (finite at the poles) */
+            leg_column_pow_(c, mcol, c->ct[k], c->st[k], ltop, y, (m == 0) ? 0 : 1);
+            const double s1 = 1.0;
             double ver = 0, vei = 0, vor = 0, voi = 0, wer = 0, wei = 0, wor = 0, woi = 0;
             for (int l = mcol; l <= ltop; ++l) {
                 const int j = l - mcol;
@@ -501,8 +554,8 @@ void orc_spat_to_SHsphtor(const orc_cfg *c, const double *Vt, const double *Vp, 
         double *w = (double *)calloc(2 * (size_t)(n + 1), sizeof(double));
         double *y = (double *)malloc(sizeof(double) * (n + 1));
         for (int k = c->tm[im]; k < nl2; ++k) {
-            leg_column_m_(c, mcol, c->ct[k], c->st[k], ltop, y);
-            const double wk = c->w[k] * wfac * ((m == 0) ? 1.0 : 1.0 / c->st[k]);
+            leg_column_pow_(c, mcol, c->ct[k], c->st[k], ltop, y, (m == 0) ? 0 : 1);
+            const double wk = c->wp[k] * wfac;
             const double *Fn = Ft + 2 * ((size_t)k * mm1 + im), *Fs = Ft + 2 * ((size_t)(nlat - 1 - k) * mm1 + im);
             const double *Gn = Fp + 2 * ((size_t)k * mm1 + im), *Gs = Fp + 2 * ((size_t)(nlat - 1 - k) * mm1 + im);
             double ter = (Fn[0] + Fs[0]) * wk, tei = (Fn[1] + Fs[1]) * wk;

@@ -23,8 +23,11 @@ extern "C" {
 
 typedef struct orc_cfg orc_cfg;
 
-/* lmax>=1, mmax*mres<=lmax, nlat even > lmax, nphi > 2*mmax. Returns NULL on bad sizes. */
+/* lmax>=1, mmax*mres<=lmax, nlat > lmax, nphi > 2*mmax. Returns NULL on bad sizes.
+ * grid: 0 Gauss (orc_create), 1 regular without poles (Fejer), 2 regular with both poles
+ * (Clenshaw-Curtis).  Odd nlat puts a row on the equator. */
 orc_cfg *orc_create(int lmax, int mmax, int mres, int nlat, int nphi);
+orc_cfg *orc_create_grid(int lmax, int mmax, int mres, int nlat, int nphi, int grid);
 void orc_destroy(orc_cfg *c);
 long orc_nlm(const orc_cfg *c);
 void orc_set_threads(int n);

@@ -32,7 +32,7 @@ SHT_LOAD_SAVE_CFG = 256 * 64
 SHT_ALLOW_GPU = 256 * 128
 SHT_ALLOW_PADDING = 256 * 256
 # contraction precision of the GPU Legendre step (shtns_hip.h)
-SHT_PREC_AUTO, SHT_PREC_F64, SHT_PREC_F32, SHT_PREC_SPLIT = -1, 0, 1, 2
+SHT_PREC_AUTO, SHT_PREC_F64, SHT_PREC_F32, SHT_PREC_SPLIT, SHT_PREC_SLICED = -1, 0, 1, 2, 3
 
 __all__ = ["sht", "rotation", "nlm_calc", "build_info"]
 
@@ -123,7 +123,9 @@ class sht:
         return self.nlat, self.nphi
 
     def gauss_wts(self):
-        w = np.empty(self.nlat // 2)
+        """Quadrature weights of the latitude pairs, north to equator ((nlat+1)/2 values; the
+        equator row of an odd grid carries half its weight, as in SHTns)."""
+        w = np.empty((self.nlat + 1) // 2)
         _lib.load().shtns_gauss_wts(self._cfg, _ptr(w))
         return w
 

@@ -133,6 +133,34 @@ static void fejer1_grid(double *x, double *st, double *w, int n) {
     }
 }
 
+// Regular grid including both poles: theta_k = k pi / N, k = 0..N (N = n - 1), with the
+// Clenshaw-Curtis rule  w_k = (c_k / N) [1 - sum_{j=1}^{N/2} b_j cos(2 j theta_k) / (4 j^2 - 1)],
+// c_0 = c_N = 1, c_k = 2 otherwise, b_j = 1 for j = N/2 and 2 otherwise (Waldvogel, BIT 46
+// (2006)); exact for polynomials in cos(theta) of degree <= N.  Cosines from the Chebyshev
+// recurrence in long double.
+static void clenshaw_curtis_grid(double *x, double *st, double *w, int n) {
+    const long double pi = 3.141592653589793238462643383279502884L;
+    const int N = n - 1;
+    for (int k = 0; k < (n + 1) / 2; ++k) {
+        const long double th = pi * k / N;
+        const long double c2 = cosl(2 * th);
+        long double tm = 1.0L, t = c2, sum = 0.0L;
+        for (int j = 1; j <= N / 2; ++j) {
+            const long double b = (2 * j == N) ? 1.0L : 2.0L;
+            sum += b * t / (4.0L * j * j - 1.0L);
+            const long double tn = 2 * c2 * t - tm;
+            tm = t;
+            t = tn;
+        }
+        const long double ck = (k == 0) ? 1.0L : 2.0L;
+        w[k] = w[n - 1 - k] = (double)(ck / N * (1.0L - sum));
+        x[k] = (k == 0) ? 1.0 : (double)cosl(th);
+        x[n - 1 - k] = -x[k];
+        st[k] = st[n - 1 - k] = (k == 0) ? 0.0 : (double)sinl(th);
+    }
+    if (n & 1) { x[n / 2] = 0.0; st[n / 2] = 1.0; }
+}
+
 // Gauss-Legendre nodes x_k = cos(theta_k) (descending, north first) and weights (sum 2).
 // Each root of P_n is found by Newton's method on the colatitude, in long double:
 //   f(theta) = P_n(cos theta),  f'(theta) = n (cos(theta) P_n - P_{n-1}) / sin(theta),
@@ -171,6 +199,17 @@ static void gauss_nodes(double *x, double *st, double *w, int n) {
         x[n - 1 - k] = (double)-c;
         st[k] = st[n - 1 - k] = (double)s;
         w[k] = w[n - 1 - k] = (double)wk;
+    }
+    if (n & 1) {   // odd n: the root x = 0, weight 2 / (n P_{n-1}(0))^2
+        long double a = 1.0L, b = 0.0L;   // P_0(0), P_1(0)
+        for (int l = 1; l < n - 1; ++l) {
+            const long double nx = -(long double)l * a / (l + 1);   // P_{l+1}(0) = -l P_{l-1}(0) / (l+1)
+            a = b;
+            b = nx;
+        }
+        x[n / 2] = 0.0;
+        st[n / 2] = 1.0;
+        w[n / 2] = (double)(2.0L / ((n * b) * (n * b)));
     }
 }
 
@@ -249,7 +288,8 @@ static void free_device(shtns_info *c) {
     (void)hipSetDevice(c->device);
     if (c->clone_of == nullptr) {
         DeviceTables &d = c->d;
-        void *ptrs[] = {d.x, d.x2, d.st, d.wq, d.step_ofs, d.ab, d.pm, d.tm, d.i0, d.z0, d.roots, d.fft_pos};
+        void *ptrs[] = {d.x, d.x2, d.st, d.wq, d.step_ofs, d.ab, d.pm, d.tm, d.i0, d.z0, d.i0v, d.z0v, d.roots, d.fft_pos,
+                        d.sb_ofs, d.eA, d.abg, d.delta};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     }
@@ -444,8 +484,10 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     const int grid = (int)flags & 255;
     const int layout = (int)flags & ~255;
     const bool regular = (grid == sht_reg_fast || grid == sht_reg_dct);   // Fejer grid, no poles
-    if (!regular && grid != sht_gauss && grid != sht_auto && grid != sht_quick_init && grid != sht_gauss_fly) {
-        set_error(1, "grid type %d is not implemented (Gauss and regular grids without poles are)", grid);
+    const bool poles = (grid == sht_reg_poles);                            // Clenshaw-Curtis, both poles
+    if (!regular && !poles && grid != sht_gauss && grid != sht_auto && grid != sht_quick_init &&
+        grid != sht_gauss_fly) {
+        set_error(1, "grid type %d is not implemented", grid);
         return 0;
     }
     if (c->clone_of != nullptr) {
@@ -459,8 +501,9 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     if (nl_order <= 0) nl_order = 1;
     if (*nlat <= 0) {   // choose sizes like SHTns: Gauss nlat > (nl_order+1)*lmax/2, regular
                         // nlat >= (nl_order+1)*(lmax+1); nphi > (nl_order+1)*mmax
-        int n = regular ? (nl_order + 1) * (c->lmax + 1) : ((nl_order + 1) * c->lmax) / 2 + 2;
+        int n = (regular || poles) ? (nl_order + 1) * (c->lmax + 1) : ((nl_order + 1) * c->lmax) / 2 + 2;
         n += (n & 1);
+        if (poles) n += 1;   // 2^k + 1 style: an odd count puts a row on the equator
         *nlat = n;
     }
     if (*nphi <= 0) {
@@ -475,10 +518,6 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
         *nphi = n;
     }
     if (*nlat <= c->lmax) runerr("Nlat must be larger than Lmax");
-    if (*nlat & 1) {
-        set_error(1, "odd nlat is not implemented (nlat=%d)", *nlat);
-        return 0;
-    }
     if (*nphi <= 2 * c->mmax) runerr("Nphi must be larger than 2*Mmax");
     if (*nphi & 1) {
         set_error(1, "odd nphi is not implemented (nphi=%d)", *nphi);
@@ -491,7 +530,7 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     }
     shtns_unset_grid(c);
     c->nlat = *nlat;
-    c->nlat_2 = (unsigned short)(*nlat / 2);
+    c->nlat_2 = (unsigned short)((*nlat + 1) / 2);   // an odd grid's equator row is its own mirror
     c->nphi = *nphi;
     c->nspat = c->nlat * c->nphi;
     c->nlat_padded = c->nlat;
@@ -502,17 +541,20 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     c->st = c->ct + c->nlat;
     std::vector<double> w(c->nlat);
     if (regular) fejer1_grid(c->ct, c->st, w.data(), c->nlat);
+    else if (poles) clenshaw_curtis_grid(c->ct, c->st, w.data(), c->nlat);
     else gauss_nodes(c->ct, c->st, w.data(), c->nlat);
     {   // quadrature sanity sums (sht_init.c:607-624)
         double s = 0, x2 = 0, st2 = 0;
-        for (int i = 0; i < c->nlat_2; ++i) {
-            const int i2 = c->nlat - 1 - i;
-            s += w[i] + w[i2];
-            x2 += w[i] * c->ct[i] * c->ct[i] + w[i2] * c->ct[i2] * c->ct[i2];
-            st2 += w[i] * c->st[i] * c->st[i] + w[i2] * c->st[i2] * c->st[i2];
+        for (unsigned i = 0; i < c->nlat; ++i) {
+            s += w[i];
+            x2 += w[i] * c->ct[i] * c->ct[i];
+            st2 += w[i] * c->st[i] * c->st[i];
         }
         if (fabs(s - 2.0) + fabs(x2 * 1.5 - 1.) + fabs(st2 * 0.75 - 1.) > 1e-14) runerr("Bad quadrature accuracy.");
     }
+    // stored per latitude pair; the equator row of an odd grid is both rows of its pair, so
+    // its weight is halved (sht_init.c:598)
+    if (c->nlat & 1) w[c->nlat / 2] *= 0.5;
     if (layout & SHT_SOUTH_POLE_FIRST)
         for (unsigned i = 0; i < c->nlat; ++i) c->ct[i] = -c->ct[i];
     c->wg.assign(w.begin(), w.begin() + c->nlat_2);
@@ -920,7 +962,8 @@ static int device_init(shtns_info *c) {
         return -1;
     }
     const size_t nseed = (size_t)(mmax + 2) * nl2;
-    if (hipMalloc(&d.i0, nseed * sizeof(int)) != hipSuccess || hipMalloc(&d.z0, nseed * sizeof(double2)) != hipSuccess) {
+    if (hipMalloc(&d.i0, nseed * sizeof(int)) != hipSuccess || hipMalloc(&d.z0, nseed * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&d.i0v, nseed * sizeof(int)) != hipSuccess || hipMalloc(&d.z0v, nseed * sizeof(double2)) != hipSuccess) {
         set_error(2, "device seed table allocation failed");
         c->gpu_ready = true;
         free_device(c);
@@ -930,6 +973,7 @@ static int device_init(shtns_info *c) {
     // coefficient are then below double resolution of the result); the polar threshold
     // eps acts through the per-order latitude skip c->tm only, as in SHTns
     launch_seed_tables(&d, L, mmax, mres, nl2, 1e-280, nullptr);
+    launch_seed_tables(&d, L, mmax, mres, nl2, 1e-280, nullptr, 1);
     HIP_OK_RET(hipGetLastError(), -1);
     HIP_OK_RET(hipDeviceSynchronize(), -1);
     // fixed workspace of cu_SH_to_spat_batch_noalloc: its address on this device (once per
@@ -1157,6 +1201,76 @@ static void *xscratch(shtns_cfg c, size_t bytes) {
 // V is double (vf == nullptr) or, for the FP32-data entry points, float: vf != nullptr, with
 // vstage an FP64 staging area of nfields * nspat doubles when the register FFT cannot take
 // the length (then the FFT writes vstage and a conversion kernel writes vf).
+// Tables of the sliced (int8) contraction, built on first use by the configuration that owns
+// the device tables (a clone builds them on its source and copies the pointers).
+static bool ensure_i8_tables(shtns_cfg c) {
+    shtns_info *own = c->clone_of ? c->clone_of : c;
+    DeviceTables &d = own->d;
+    if (!d.eA) {
+        const int nslot = own->mmax + 1;
+        std::vector<int> steps(nslot + 2), sbo(nslot + 1);
+        if (hipMemcpy(steps.data(), d.step_ofs, sizeof(int) * (nslot + 1), hipMemcpyDeviceToHost) != hipSuccess) {
+            set_error(2, "sliced tables: copy failed");
+            return false;
+        }
+        int tot = 0, mx = 0;
+        for (int sl = 0; sl < nslot; ++sl) {
+            sbo[sl] = tot;
+            const int nsb = (steps[sl + 1] - steps[sl] + 127) / 128;
+            tot += nsb;
+            mx = std::max(mx, nsb);
+        }
+        sbo[nslot] = tot;
+        // balanced recurrence: w_i = 2^delta_i z_i with delta_i = ilogb(d_i), d_i = uo_i
+        const int nall = own->nsteps_total;
+        std::vector<double4> abg(nall, make_double4(0.0, 0.0, 0.0, 0.0));
+        std::vector<int> dl(nall, 0);
+        for (int sl = 0; sl < nslot; ++sl) {
+            const int m = sl * own->mres, n = steps[sl + 1] - steps[sl];
+            if (n <= 0) continue;
+            std::vector<double2> ab(n);
+            std::vector<double4> pm(n);
+            build_order_tables(m, n, ab.data(), pm.data());
+            std::vector<int> de(n + 1);
+            for (int i = 0; i < n; ++i) de[i] = (pm[i].z != 0.0) ? ilogb(pm[i].z) : 0;
+            de[n] = de[n - 1];
+            for (int i = 0; i < n; ++i) {
+                const int dp = de[i + 1] - de[i], dg = de[i + 1] - de[i > 0 ? i - 1 : 0];
+                abg[steps[sl] + i] = make_double4(ldexp(ab[i].x, dp), ldexp(ab[i].y, dp), ldexp(1.0, dg), 0.0);
+                dl[steps[sl] + i] = de[i];
+            }
+        }
+        int *dsb = nullptr, *ddl = nullptr;
+        double4 *dabg = nullptr;
+        short *dea = nullptr;
+        if (upload(&dsb, sbo.data(), sbo.size()) || upload(&dabg, abg.data(), abg.size()) ||
+            upload(&ddl, dl.data(), dl.size()) ||
+            hipMalloc(&dea, sizeof(short) * (size_t)tot * own->nlat_2 + 256) != hipSuccess) {
+            for (void *p : {(void *)dsb, (void *)dabg, (void *)ddl})
+                if (p) (void)hipFree(p);
+            set_error(2, "sliced tables: allocation failed");
+            return false;
+        }
+        d.sb_ofs = dsb;
+        d.sb_total = tot;
+        d.sb_max = mx;
+        d.abg = dabg;
+        d.delta = ddl;
+        d.eA = dea;
+        launch_i8_rowexp(&d, own->mmax, own->nlat_2, c->comp_stream);
+        HIP_OK_RET(hipGetLastError(), false);
+    }
+    if (c != own) {
+        c->d.sb_ofs = d.sb_ofs;
+        c->d.sb_total = d.sb_total;
+        c->d.sb_max = d.sb_max;
+        c->d.abg = d.abg;
+        c->d.delta = d.delta;
+        c->d.eA = d.eA;
+    }
+    return true;
+}
+
 // Scratch bytes of one synth_core call (Fourier rows F, then the pre-mixed coefficients).
 static size_t synth_scratch_bytes(shtns_cfg c, int nfields) {
     const size_t fbytes = (size_t)nfields * (c->mmax + 1) * c->nlat * sizeof(double2);
@@ -1174,10 +1288,6 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
     const size_t fbytes = (size_t)nfields * four_dist * sizeof(double2);
-    char *base = ext ? (char *)ext : (char *)scratch(c, synth_scratch_bytes(c, nfields));
-    if (!base) return;
-    double2 *F = (double2 *)base;
-    double *qmix = (double *)(base + fbytes);
     LegendreArgs a = leg_args(c, ltr, imlim, nfields);
     a.mpos_scale = mpos_scale;
     a.lat_weight = lat_weight;
@@ -1185,11 +1295,37 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     long sp, st;
     spat_strides(c, sp, st);
     const hipStream_t S = c->comp_stream;
+    const bool prof = !ext;
+    if (prec == kPrecSliced && !vf && !ext) {   // int8 digit planes of the coefficients
+        if (!ensure_i8_tables(c)) return;
+        a.synth_ncg = 2;   // pre-mix 32 columns per 8 fields
+        const size_t qb = synth_qmix_doubles(nfields, c->nsteps_total, 2) * sizeof(double);
+        const size_t pb = i8_planes_bytes(nfields, c->d.sb_total), eb = i8_eB_bytes(nfields, c->d.sb_total);
+        char *b2 = (char *)scratch(c, fbytes + qb + pb + eb + 512);
+        if (!b2) return;
+        double2 *F = (double2 *)b2;
+        double *q8 = (double *)(b2 + fbytes);
+        unsigned char *pl = (unsigned char *)(((uintptr_t)(b2 + fbytes + qb) + 255) & ~(uintptr_t)255);
+        int *eB = (int *)(((uintptr_t)(pl + pb) + 255) & ~(uintptr_t)255);
+        {
+            ProfScope ps(c, PK_LEG_SYNTH, nullptr, false, prof);
+            launch_premix(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, q8, c->nsteps_total, S);
+            launch_legendre_synth_i8(a, c->lmax, F, four_dist, q8, c->nsteps_total, pl, eB, S);
+        }
+        ProfScope ps(c, PK_FFT_SYNTH, nullptr, false, prof);
+        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S, 0,
+                         -1, c->tune_fft_synth);
+        HIP_OK(hipGetLastError());
+        return;
+    }
+    char *base = ext ? (char *)ext : (char *)scratch(c, synth_scratch_bytes(c, nfields));
+    if (!base) return;
+    double2 *F = (double2 *)base;
+    double *qmix = (double *)(base + fbytes);
     const int ntiles = (c->nlat_2 + 63) / 64;
     const int chunk_tiles = 4;
     const int nchunks = (ntiles + chunk_tiles - 1) / chunk_tiles;
     hipStream_t A = (overlap_enabled() && nchunks > 1 && !vf && !ext) ? aux_stream(c) : nullptr;
-    const bool prof = !ext;
     {
         ProfScope ps(c, PK_LEG_SYNTH, nullptr, false, prof);
         launch_premix(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, qmix, c->nsteps_total, S);
@@ -1451,7 +1587,7 @@ extern "C" void cu_spat_to_SH_float(shtns_cfg c, float *V, cplx_f *Q, int ltr) {
 }
 extern "C" int cushtns_set_precision(shtns_cfg c, int prec) {
     clear_error();
-    if (c == nullptr || prec < SHT_PREC_AUTO || prec > SHT_PREC_SPLIT) {
+    if (c == nullptr || prec < SHT_PREC_AUTO || prec > SHT_PREC_SLICED) {
         set_error(1, "cushtns_set_precision: unknown precision %d", prec);
         return -1;
     }
@@ -1809,7 +1945,7 @@ static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g
     g.nq = c->lmax - g.m + 1;
     const int nlat = c->nlat;
     std::vector<double> w(nlat);
-    for (int k = 0; k < nlat; ++k) w[k] = c->wg[std::min(k, nlat - 1 - k)];
+    for (int k = 0; k < nlat; ++k) w[k] = c->wg[std::min(k, nlat - 1 - k)] * ((2 * k + 1 == nlat) ? 2.0 : 1.0);
     std::vector<double2> ab(std::max(g.nl, 2));
     const int m = (vector && g.m == 0) ? 1 : g.m;     // order of the recurrence
     ab[0] = make_double2(0.0, 0.0);

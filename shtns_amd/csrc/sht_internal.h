@@ -46,6 +46,20 @@ struct DeviceTables {
     int *tm = nullptr;         // [mmax+2] polar optimization: latitude pairs k < tm[slot] skipped
     int *i0 = nullptr;         // [(mmax+2) * nlat_2] first step with significant values
     double2 *z0 = nullptr;     // [(mmax+2) * nlat_2] (z_{i0-1}, z_{i0})
+    // the same for the vector transforms' functions P/sin(theta) (seeds from sin^(m-1), so
+    // finite at the poles); slot mmax+1 (the m=1 functions of vector m=0) as for scalars
+    int *i0v = nullptr;
+    double2 *z0v = nullptr;
+    // sliced (int8) contraction, built on first use: per order slot, super-blocks of 128
+    // recurrence steps (sb_ofs[slot] = first, sb_total in all, sb_max in one slot) and the
+    // power-of-two scale of the recurrence values per (super-block, latitude pair)
+    int *sb_ofs = nullptr;     // [mmax+2]
+    int sb_total = 0, sb_max = 0;
+    short *eA = nullptr;       // [sb_total * nlat_2]
+    // balanced recurrence w_i = 2^delta_i z_i (delta_i = ilogb(d_i)), per step like ab:
+    // (alpha', beta', gamma', 0) with w_{i+1} = (alpha' x^2 + beta') w_i + gamma' w_{i-1}
+    double4 *abg = nullptr;
+    int *delta = nullptr;
     // FFT: roots e^{2 pi i k/nphi}, k < nphi
     double2 *roots = nullptr;
     int *fft_pos = nullptr;    // [M] position of output bin k after the in-place DIF FFT
@@ -123,7 +137,7 @@ void set_error(int code, const char *fmt, ...);
 void clear_error();
 
 // Contraction precisions of the Legendre kernels (SHT_PREC_* in shtns_hip.h)
-constexpr int kPrecF64 = 0, kPrecF32 = 1, kPrecSplit = 2;
+constexpr int kPrecF64 = 0, kPrecF32 = 1, kPrecSplit = 2, kPrecSliced = 3;
 
 // Fixed device workspace of cu_SH_to_spat_batch_noalloc (sht_noalloc.hip): one array per
 // GPU, kept until the process ends.  noalloc_scratch_setup resolves the array's device
@@ -194,6 +208,13 @@ void launch_interleave(const double *xy, double2 *z, long z_dist, long n, int nf
 void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int nf, hipStream_t s);
 // Legendre transforms at one order (no FFT): table of P_lm at every latitude, sum over degrees,
 // weighted sum over latitudes
+// sliced int8 synthesis (sht_legendre_i8.hip): row-exponent table, scratch sizes, launch
+// (qmix from launch_premix with synth_ncg = 2: 32 columns per 8 fields)
+void launch_i8_rowexp(const DeviceTables *t, int mmax, int nlat_2, hipStream_t s);
+size_t i8_planes_bytes(int nfields, int sb_total);
+size_t i8_eB_bytes(int nfields, int sb_total);
+void launch_legendre_synth_i8(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
+                              int nsteps_total, unsigned char *planes, int *eB, hipStream_t s);
 // ktm: latitudes k < ktm and k >= nlat - ktm get zero tables (polar optimization of the order)
 void launch_ml_table(const double *ct, const double *st, int nlat, int m, int nl, const double2 *ab, double log2_seed,
                      double seed_sign, int ktm, double *T, hipStream_t s);
@@ -221,7 +242,9 @@ void launch_legendre_anal_f32(const LegendreArgs &a, const double2 *F, long four
                               int spec_ltop, hipStream_t s);
 void launch_legendre_anal_split(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Qlm,
                                 long spec_dist, int spec_ltop, hipStream_t s);
-void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s);
+// seeds of the scalar functions (vec = 0: t->i0, t->z0) or of the vector ones (vec = 1)
+void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s,
+                        int vec = 0);
 void launch_sphtor_to_vw(const double2 *S, const double2 *T, long spec_dist, double2 *VW, long vw_dist,
                          int lmax, int mmax, int mres, int ltr, int imlim, int nfields, hipStream_t s);
 void launch_vw_to_sphtor(const double2 *VW, long vw_dist, double2 *S, double2 *T, long spec_dist,

@@ -42,7 +42,7 @@ namespace shtns_amd {
 __global__ void seed_kernel(const double *__restrict__ xk, const double *__restrict__ stk,
                             const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
                             const double4 *__restrict__ pm, const int *__restrict__ tm, int lmax, int mmax,
-                            int mres, int nlat_2, double thr, int *__restrict__ i0_out,
+                            int mres, int nlat_2, double thr, int vec, int *__restrict__ i0_out,
                             double2 *__restrict__ z0_out) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
@@ -59,13 +59,13 @@ __global__ void seed_kernel(const double *__restrict__ xk, const double *__restr
     const double x = xk[k];
     const double x2 = x * x;
     const double s = stk[k];
-    // sin^m as mant * 2^E
+    // sin^m as mant * 2^E (sin^(m-1) for the vector functions P/sin of orders m >= 1)
     int E = 0, es;
     double mant = 1.0;
     {
         double b = frexp(s, &es);
         int eb = es;
-        int n = m;
+        int n = (vec && slot <= mmax && m >= 1) ? m - 1 : m;
         while (n) {
             if (n & 1) {
                 mant *= b;
@@ -114,10 +114,11 @@ __global__ void seed_kernel(const double *__restrict__ xk, const double *__restr
     (void)lmax;
 }
 
-void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s) {
+void launch_seed_tables(DeviceTables *t, int lmax, int mmax, int mres, int nlat_2, double thr, hipStream_t s,
+                        int vec) {
     dim3 grid((nlat_2 + 63) / 64, mmax + 2);
     hipLaunchKernelGGL(seed_kernel, grid, dim3(64), 0, s, t->x, t->st, t->step_ofs, t->ab, t->pm, t->tm, lmax,
-                       mmax, mres, nlat_2, thr, t->i0, t->z0);
+                       mmax, mres, nlat_2, thr, vec, vec ? t->i0v : t->i0, vec ? t->z0v : t->z0);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -411,7 +412,14 @@ __global__ void ml_vtable_kernel(const double *__restrict__ ct, const double *__
         return;
     }
     const int mr = (m >= 1) ? m : 1;     // order of the recurrence run here
-    const double lg = log2_seed + (mr - 1) * log2(s);
+    if (mr > 1 && !(s > 0.0)) {           // P/sin(theta) vanishes at the poles for orders >= 2
+        for (int o = 0; o < nl; ++o) {
+            Y[(size_t)o * nlat + k] = 0.0;
+            D[(size_t)o * nlat + k] = 0.0;
+        }
+        return;
+    }
+    const double lg = log2_seed + ((mr > 1) ? (mr - 1) * log2(s) : 0.0);
     int e = (int)floor(lg);
     double y0 = seed_sign * exp2(lg - e), y1 = 0.0;
     // j counts degrees l = mr + j of the recurrence; output index o = l - m

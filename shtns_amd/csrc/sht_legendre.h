@@ -31,7 +31,7 @@ struct OrderInfo {
     int mfun;       // order of the generated functions
     int nsteps;     // recurrence steps to evaluate (degrees mfun .. ltr)
     int spec_base;  // index of degree 0 of this order in the spectral layout (may be virtual)
-    bool sinv;      // divide functions by sin(theta) (vector V,W spectra, m>0)
+    bool sinv;      // divide the seeds by sin(theta) (unused: vector seeds have their own table)
     bool real_only; // m = 0: imaginary parts ignored / zero
 };
 
@@ -46,7 +46,7 @@ __device__ inline OrderInfo order_info(int im, int lmax_spec, int mmax, int mres
     } else {
         o.slot = im;
         o.mfun = m;
-        o.sinv = (vector_mode != 0);
+        o.sinv = false;   // vector mode reads seeds of P/sin(theta) (from sin^(m-1), finite at the poles)
     }
     o.nsteps = (ltr >= o.mfun) ? (ltr - o.mfun) / 2 + 1 : 0;
     o.spec_base = (int)(((long)im * (2 * lmax_spec + 2 - (im + 1) * mres)) / 2);
@@ -299,7 +299,8 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
                 const int kc = min(kr, nlat_2 - 1);
                 const double x = xk[kc];
                 xs[j] = o_lane ? -x : x;
-                wz[j] = zero_im ? 0.0 : (lat_weight ? wqk[kc] : 1.0);
+                // the middle row of an odd grid is its own mirror: its stored pair weight is halved
+                wz[j] = zero_im ? 0.0 : (lat_weight ? wqk[kc] * ((2 * kr + 1 == nlat) ? 2.0 : 1.0) : 1.0);
                 row[j] = o_lane ? (nlat - 1 - kr) : kr;
                 ok[j] = kr < nlat_2;
             }
@@ -337,7 +338,8 @@ void synth_tiles_launch(const LegendreArgs &a, int spec_ltop, double2 *F, long f
     dim3 grid((a.mmax + 1) * ntg, ng);
 #define SYNTH_LAUNCH(NC)                                                                                         \
     hipLaunchKernelGGL((legendre_synth_kernel<NC, PREC>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq, \
-                       t->step_ofs, t->ab, t->i0, t->z0, qmix, nsteps_total, spec_ltop, F, four_dist, a.mmax,    \
+                       t->step_ofs, t->ab, a.vector_mode ? t->i0v : t->i0, a.vector_mode ? t->z0v : t->z0, qmix, \
+                       nsteps_total, spec_ltop, F, four_dist, a.mmax,                                              \
                        a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.lat_weight, tile0, ntl)
     if (ncg == 4) SYNTH_LAUNCH(4);
     else if (ncg == 2) SYNTH_LAUNCH(2);
@@ -448,7 +450,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
                     double v = 0.0;
                     if (kv) {
                         const double2 fn = Ff[kl], fs = Ff[nlat - 1 - kl];
-                        const double w = lat_weight ? wqk[kl] : 1.0;
+                        const double w = lat_weight ? wqk[kl] : ((2 * kl + 1 == nlat) ? 0.5 : 1.0);   // odd grid: equator row once
                         v = (comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
                              : comp == 2 ? (fn.x - fs.x) * xk[kl] : (fn.y - fs.y) * xk[kl]) * w;
                     }
@@ -457,7 +459,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
                     const int klc = kv ? kl : 0;
                     const double2 fn = Ff[klc];
                     const double2 fs = Ff[nlat - 1 - klc];
-                    const double w = lat_weight ? wqk[klc] : 1.0;
+                    const double w = lat_weight ? wqk[klc] : ((2 * klc + 1 == nlat) ? 0.5 : 1.0);
                     const double x = xk[klc];
                     const double v = comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
                                    : comp == 2 ? (fn.x - fs.x) * x : (fn.y - fs.y) * x;
@@ -647,7 +649,8 @@ void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, doub
     const size_t dyn = cf_lds ? cf_bytes : 0;
     ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC>, dyn);
     hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC>), grid, dim3(64 * NW), dyn, s, t->x, t->x2, t->st,
-                       t->wq, t->step_ofs, t->ab, t->pm, t->i0, t->z0, F, four_dist, Q, spec_dist,
+                       t->wq, t->step_ofs, t->ab, t->pm, a.vector_mode ? t->i0v : t->i0,
+                       a.vector_mode ? t->z0v : t->z0, F, four_dist, Q, spec_dist,
                        spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
                        a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale, a.lat_weight, cf_lds);
 }

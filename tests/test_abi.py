@@ -107,8 +107,10 @@ def test_sizes_validation_and_errors_without_gpu():
     s = shtns_amd.sht(31)
     with pytest.raises(ValueError):
         s.set_grid(30, 64)       # nlat <= lmax
+    s.set_grid(33, 64)           # odd nlat: the equator row is its own mirror
+    assert s.nlat == 33 and s.gauss_wts().size == 17
     with pytest.raises(RuntimeError):
-        s.set_grid(33, 64)       # odd nlat not implemented
+        s.set_grid(32, 66)       # nphi/2 = 33 = 3 * 11 has a factor the FFT does not take
     s.set_grid(32, 64)
     if not gpu_available():
         with pytest.raises(RuntimeError, match="no usable GPU"):

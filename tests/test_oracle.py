@@ -318,3 +318,59 @@ def test_polar_skip_semantics():
         qm = o0.analys(vm)
         sl = slice(o0.idx(m, m), o0.idx(m, m) + L + 1 - m)
         assert rel(qe[sl], qm[sl]) <= 1e-13
+
+
+# ---------------------------------------------------------------- other grids
+# sht_reg_poles (Clenshaw-Curtis with both poles), Fejer without poles, odd nlat (an equator
+# row), pinned the same way: brute-force scipy sums at the grid points, round trips, and for
+# the vector functions at the poles the limit of the analytic field at theta -> 0, pi.
+@pytest.mark.parametrize("case", [(20, 20, 1, 41, 48, Oracle.REGULAR_POLES), (20, 20, 1, 42, 48, Oracle.REGULAR_POLES),
+                                  (24, 12, 2, 25, 30, Oracle.GAUSS), (20, 20, 1, 45, 44, Oracle.REGULAR)])
+def test_synth_other_grids_vs_bruteforce(case):
+    lmax, mmax, mres, nlat, nphi, grid = case
+    o = Oracle(lmax, mmax, mres, nlat, nphi, grid=grid)
+    q = spectral_field(lmax, mmax, seed=12, mres=mres)
+    th = np.arccos(np.clip(o.cos_theta, -1, 1))
+    ph = 2 * np.pi * np.arange(nphi) / (nphi * mres)
+    assert rel(o.synth(q), _brute_synth(o, q, th, ph)) < 1e-14
+    if grid == Oracle.REGULAR_POLES:
+        assert o.cos_theta[0] == 1.0 and o.cos_theta[-1] == -1.0
+    # exact quadrature for band-limited data: nlat > 2 lmax for the regular grids
+    assert rel(o.analys(o.synth(q)), q) < 1e-13
+
+
+def _vector_analytic(o, s, t, th, ph):
+    TH, PH = np.meshgrid(th, ph)
+    vt = np.zeros_like(TH)
+    vp = np.zeros_like(TH)
+    for l in range(1, o.lmax + 1):
+        for m in range(0, l + 1):
+            f = 1.0 if m == 0 else 2.0
+            dY = _dtheta_Y(l, m, TH, PH)
+            Y = sph_harm_y(l, m, TH, PH)
+            cs, ct = s[o.idx(l, m)], t[o.idx(l, m)]
+            vt += f * np.real(cs * dY + ct * 1j * m * Y / np.sin(TH))
+            vp += f * np.real(cs * 1j * m * Y / np.sin(TH) - ct * dY)
+    return vt, vp
+
+
+@pytest.mark.parametrize("nlat", [41, 40])
+def test_vector_reg_poles(nlat):
+    L, nphi = 15, 32
+    o = Oracle(L, L, 1, nlat, nphi, grid=Oracle.REGULAR_POLES)
+    s = spectral_field(L, L, seed=42, power=3.0)
+    t = spectral_field(L, L, seed=43, power=3.0)
+    s[0] = t[0] = 0
+    th = np.arccos(np.clip(o.cos_theta, -1, 1))
+    ph = 2 * np.pi * np.arange(nphi) / nphi
+    vt_o, vp_o = o.synth_sphtor(s, t)
+    inner = slice(1, nlat - 1)
+    vt, vp = _vector_analytic(o, s, t, th[inner], ph)
+    assert rel(vt_o[:, inner], vt) < 1e-13 and rel(vp_o[:, inner], vp) < 1e-13
+    # pole rows: the limit of the field (theta = 1e-7 from each pole; the field is smooth)
+    for row, tp in ((0, 1e-7), (nlat - 1, np.pi - 1e-7)):
+        lt, lp = _vector_analytic(o, s, t, np.array([tp]), ph)
+        assert np.max(np.abs(vt_o[:, row] - lt[:, 0])) < 1e-5 * np.max(np.abs(vt))
+        assert np.max(np.abs(vp_o[:, row] - lp[:, 0])) < 1e-5 * np.max(np.abs(vp))
+    s2, t2 = o.analys_sphtor(vt_o, vp_o)
+    assert rel(s2, s) < 1e-13 and rel(t2, t) < 1e-13

@@ -4,7 +4,7 @@ import numpy as np
 
 from oracle.oracle import Oracle
 from shtns_amd.synthetic import spectral_field
-from sampled_oracle import anal_orders, spat_rows, synth_rows
+from sampled_oracle import anal_orders, anal_orders_ld, spat_rows, synth_rows, synth_rows_ld
 
 
 def rel(a, b):
@@ -32,3 +32,16 @@ def test_anal_orders_match_full_analysis():
             m = im * mres
             b = o.idx(m, m)
             assert rel(q, ref[b:b + L + 1 - m]) <= 1e-13
+
+
+def test_extended_precision_references_match_oracle():
+    L, nlat, nphi = 63, 96, 192
+    o = Oracle(L, L, 1, nlat, nphi)
+    q = spectral_field(L, L, seed=5)
+    lats = [0, 7, 47, 48, 95]
+    assert rel(synth_rows_ld(o, q, lats), synth_rows(o, q, lats)) <= 1e-14
+    V = np.random.default_rng(6).standard_normal((nphi, nlat))
+    ref = anal_orders(o, V, [0, 3, 63])
+    got = anal_orders_ld(o, V, [0, 3, 63])
+    for m in ref:
+        assert rel(got[m], ref[m]) <= 1e-14
