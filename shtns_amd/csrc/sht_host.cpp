@@ -289,7 +289,7 @@ static void free_device(shtns_info *c) {
     if (c->clone_of == nullptr) {
         DeviceTables &d = c->d;
         void *ptrs[] = {d.x, d.x2, d.st, d.wq, d.step_ofs, d.ab, d.pm, d.tm, d.i0, d.z0, d.i0v, d.z0v, d.roots, d.fft_pos,
-                        d.sb_ofs, d.eA, d.abg, d.delta};
+                        d.sb_ofs, d.eA, d.abk, d.dblk, d.delta};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     }
@@ -1221,9 +1221,13 @@ static bool ensure_i8_tables(shtns_cfg c) {
             mx = std::max(mx, nsb);
         }
         sbo[nslot] = tot;
-        // balanced recurrence: w_i = 2^delta_i z_i with delta_i = ilogb(d_i), d_i = uo_i
+        // balanced recurrence: w_i = 2^D(b, i & 1) z_i in the frame of K block b (32 steps),
+        // D(b, p) = ilogb(d) of the middle step of parity p in the block (d_i = uo_i); within a
+        // block the z_{i-1} coefficient is then exactly 1, and the state changes frame by a
+        // power of two at each block start
         const int nall = own->nsteps_total;
-        std::vector<double4> abg(nall, make_double4(0.0, 0.0, 0.0, 0.0));
+        std::vector<double2> abk((size_t)tot * 128, make_double2(0.0, 0.0));
+        std::vector<int2> dbk((size_t)tot * 4, make_int2(0, 0));
         std::vector<int> dl(nall, 0);
         for (int sl = 0; sl < nslot; ++sl) {
             const int m = sl * own->mres, n = steps[sl + 1] - steps[sl];
@@ -1231,22 +1235,31 @@ static bool ensure_i8_tables(shtns_cfg c) {
             std::vector<double2> ab(n);
             std::vector<double4> pm(n);
             build_order_tables(m, n, ab.data(), pm.data());
-            std::vector<int> de(n + 1);
-            for (int i = 0; i < n; ++i) de[i] = (pm[i].z != 0.0) ? ilogb(pm[i].z) : 0;
-            de[n] = de[n - 1];
-            for (int i = 0; i < n; ++i) {
-                const int dp = de[i + 1] - de[i], dg = de[i + 1] - de[i > 0 ? i - 1 : 0];
-                abg[steps[sl] + i] = make_double4(ldexp(ab[i].x, dp), ldexp(ab[i].y, dp), ldexp(1.0, dg), 0.0);
-                dl[steps[sl] + i] = de[i];
+            auto ilog = [&](int i) { return (pm[i].z != 0.0) ? ilogb(pm[i].z) : 0; };
+            const int nb = (n + 31) / 32;
+            for (int b = 0; b < nb; ++b) {
+                int dp[2];
+                for (int p = 0; p < 2; ++p) {
+                    int i = 32 * b + 16 + p;
+                    while (i >= n && i - 2 >= 32 * b) i -= 2;
+                    dp[p] = ilog(std::min(i, n - 1));
+                }
+                dbk[(size_t)sbo[sl] * 4 + b] = make_int2(dp[0], dp[1]);
+                for (int i = 32 * b; i < std::min(32 * b + 32, n); ++i) {
+                    const int sh = dp[(i + 1) & 1] - dp[i & 1];
+                    abk[(size_t)sbo[sl] * 128 + i] = make_double2(ldexp(ab[i].x, sh), ldexp(ab[i].y, sh));
+                    dl[steps[sl] + i] = dp[i & 1];
+                }
             }
         }
         int *dsb = nullptr, *ddl = nullptr;
-        double4 *dabg = nullptr;
+        double2 *dabk = nullptr;
+        int2 *ddbk = nullptr;
         short *dea = nullptr;
-        if (upload(&dsb, sbo.data(), sbo.size()) || upload(&dabg, abg.data(), abg.size()) ||
-            upload(&ddl, dl.data(), dl.size()) ||
+        if (upload(&dsb, sbo.data(), sbo.size()) || upload(&dabk, abk.data(), abk.size()) ||
+            upload(&ddbk, dbk.data(), dbk.size()) || upload(&ddl, dl.data(), dl.size()) ||
             hipMalloc(&dea, sizeof(short) * (size_t)tot * own->nlat_2 + 256) != hipSuccess) {
-            for (void *p : {(void *)dsb, (void *)dabg, (void *)ddl})
+            for (void *p : {(void *)dsb, (void *)dabk, (void *)ddbk, (void *)ddl})
                 if (p) (void)hipFree(p);
             set_error(2, "sliced tables: allocation failed");
             return false;
@@ -1254,7 +1267,8 @@ static bool ensure_i8_tables(shtns_cfg c) {
         d.sb_ofs = dsb;
         d.sb_total = tot;
         d.sb_max = mx;
-        d.abg = dabg;
+        d.abk = dabk;
+        d.dblk = ddbk;
         d.delta = ddl;
         d.eA = dea;
         launch_i8_rowexp(&d, own->mmax, own->nlat_2, c->comp_stream);
@@ -1264,7 +1278,8 @@ static bool ensure_i8_tables(shtns_cfg c) {
         c->d.sb_ofs = d.sb_ofs;
         c->d.sb_total = d.sb_total;
         c->d.sb_max = d.sb_max;
-        c->d.abg = d.abg;
+        c->d.abk = d.abk;
+        c->d.dblk = d.dblk;
         c->d.delta = d.delta;
         c->d.eA = d.eA;
     }
@@ -1296,8 +1311,9 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
     spat_strides(c, sp, st);
     const hipStream_t S = c->comp_stream;
     const bool prof = !ext;
-    if (prec == kPrecSliced && !vf && !ext) {   // int8 digit planes of the coefficients
-        if (!ensure_i8_tables(c)) return;
+    if (prec == kPrecSliced && !vf && !ext && !ensure_i8_tables(c)) return;
+    // int8 digit planes of the coefficients (lmax <= 4095; above, the FP64 contraction runs)
+    if (prec == kPrecSliced && !vf && !ext && i8_synth_lds_bytes(c->d.sb_max) <= kI8MaxLds) {
         a.synth_ncg = 2;   // pre-mix 32 columns per 8 fields
         const size_t qb = synth_qmix_doubles(nfields, c->nsteps_total, 2) * sizeof(double);
         const size_t pb = i8_planes_bytes(nfields, c->d.sb_total), eb = i8_eB_bytes(nfields, c->d.sb_total);

@@ -56,9 +56,12 @@ struct DeviceTables {
     int *sb_ofs = nullptr;     // [mmax+2]
     int sb_total = 0, sb_max = 0;
     short *eA = nullptr;       // [sb_total * nlat_2]
-    // balanced recurrence w_i = 2^delta_i z_i (delta_i = ilogb(d_i)), per step like ab:
-    // (alpha', beta', gamma', 0) with w_{i+1} = (alpha' x^2 + beta') w_i + gamma' w_{i-1}
-    double4 *abg = nullptr;
+    // balanced recurrence w_i = 2^D(b, i & 1) z_i in the frame of 32-step block b:
+    // w_{i+1} = (alpha' x^2 + beta') w_i + w_{i-1} inside a block; abk[sb_ofs[slot] * 128 + i] =
+    // (alpha', beta') (zero-padded to whole super-blocks), dblk[sb_ofs[slot] * 4 + b] =
+    // (D(b, 0), D(b, 1)), delta[step_ofs[slot] + i] = D(block of i, i & 1)
+    double2 *abk = nullptr;
+    int2 *dblk = nullptr;
     int *delta = nullptr;
     // FFT: roots e^{2 pi i k/nphi}, k < nphi
     double2 *roots = nullptr;
@@ -213,6 +216,10 @@ void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int 
 void launch_i8_rowexp(const DeviceTables *t, int mmax, int nlat_2, hipStream_t s);
 size_t i8_planes_bytes(int nfields, int sb_total);
 size_t i8_eB_bytes(int nfields, int sb_total);
+// LDS of the sliced synthesis kernel (sums and one order's recurrence coefficients); the
+// sliced path runs when this is within kI8MaxLds
+size_t i8_synth_lds_bytes(int sb_max);
+constexpr size_t kI8MaxLds = 80 * 1024;
 void launch_legendre_synth_i8(const LegendreArgs &a, int spec_ltop, double2 *F, long four_dist, const double *qmix,
                               int nsteps_total, unsigned char *planes, int *eB, hipStream_t s);
 // ktm: latitudes k < ktm and k >= nlat - ktm get zero tables (polar optimization of the order)

@@ -4,7 +4,7 @@
 // Both operands of the contraction F[lat][col] = sum_i z_i(lat) b_i(col) are cut into five
 // signed 8-bit digits with a power-of-two scale per latitude and per column that changes
 // every 128 recurrence steps (a "super-block"):
-//     z = 2^(eA-31) (d0 2^24 + d1 2^16 + d2 2^8 + d3 + d4 2^-8),   |z| < 2^(eA-1),
+//     z = 2^(eA-39) (d0 2^32 + d1 2^24 + d2 2^16 + d3 2^8 + d4),   |z| < 2^(eA-1),
 // and likewise b with eB.  The products of digit planes p, q with p + q <= 5 (19 of the 25)
 // are summed exactly in int32 by v_mfma_i32_32x32x32_i8 (|sum| < 2^24 per super-block), one
 // accumulator per diagonal p + q, and converted to FP64 with weight 2^(eA+eB-14-8(p+q)) at the
@@ -12,10 +12,10 @@
 // one.  The recurrence, its seeds and tables, the FFT and all sums across super-blocks stay
 // FP64; only the contraction changes.
 //
-// Digits: t = z 2^(31-eA) (|t| < 2^30) is rounded with the 1.5*2^52 trick; the low word of
-// t + 1.5*2^52 is h = rint(t) in two's complement, and (h + 0x80808080) ^ 0x80808080 holds
-// the four balanced base-256 digits of h (d0 in the top byte).  The fifth digit is
-// rint(256 (t - h)) clamped to [-128, 127].
+// Digits: t = z 2^(39-eA) (|t| < 2^38) plus the constant M = 1.5*2^52 + 0x8080808080 is
+// rounded to an integer by one FP64 add; the low 40 bits of the sum's bit pattern are then
+// rint(t) + 0x8080808080, whose bytes k = 0..4 are the balanced base-256 digits of rint(t)
+// plus 128 (each byte XOR 0x80 is the signed digit).
 //
 // Balancing: the recurrence values z_i alternate in size with the parity of i (z_i = v_i / d_i
 // with d_i d_{i+1} ~ 4 / (e e'), ratios up to ~10^3 near l ~ m), and the pre-mixed
@@ -26,13 +26,14 @@
 // g'_i = 2^(delta_{i+1}-delta_{i-1}); being exact powers of two, these reproduce 2^(delta) z
 // bit for bit.  The coefficient planes are cut from 2^(-delta_i) b_i.
 //
-// Work decomposition: one wave = 32 latitude pairs x 16 fields (two 32-column tiles of
+// Work decomposition: one wave = 32 latitude pairs x 8 fields (one 32-column tile of
 // 8 fields x (E.re, E.im, O.re, O.im)); lane l holds latitude l & 31 and, for each 32-step
 // K block, the steps of parity l >> 5 (byte j <-> step 2j + (l >> 5); the coefficient planes
 // use the same map).  Both lane halves run the latitude's recurrence; each keeps every other
-// value.  Super-block scales of the recurrence values come from a table built once per grid
-// (their magnitudes do not depend on the data); the recurrence state is rescaled by the
-// exact power of two at each super-block boundary, so t is the recurrence value itself.
+// value.  Two waves share a SIMD, so one wave's MFMAs run under the other's VALU work.
+// Super-block scales of the recurrence values come from a table built once per grid (their
+// magnitudes do not depend on the data); the recurrence state is rescaled by the exact power
+// of two at each super-block boundary, so t is the recurrence value itself.
 #include "sht_legendre.h"
 
 namespace shtns_amd {
@@ -48,17 +49,12 @@ constexpr int kI8Waves = 4;                // waves per block
 constexpr int kI8ND = 6;                   // diagonals p + q = 0..5
 constexpr int kI8Planes = 5;
 
-__device__ __forceinline__ int lo_word(double v) { return __double2loint(v); }
-
-// balanced digits of t (|t| < 2^30): packed top four digits (d0 in byte 3) and the fifth
-__device__ __forceinline__ void slice5(double t, unsigned &p4, int &d4) {
-    const double M = 6755399441055744.0;   // 1.5 * 2^52
+// t (|t| < 2^38) -> rint(t) + 0x8080808080 in the low 40 bits of (hi:lo)
+__device__ __forceinline__ void slice40(double t, unsigned &lo, unsigned &hi) {
+    constexpr double M = 6755399441055744.0 + (double)0x8080808080ull;   // 1.5 * 2^52 + bias
     const double tm = t + M;
-    const int h = lo_word(tm);
-    const double r = t - (tm - M);
-    const int e = lo_word(fma(r, 256.0, M));
-    d4 = min(max(e, -128), 127);
-    p4 = ((unsigned)h + 0x80808080u) ^ 0x80808080u;
+    lo = (unsigned)__double2loint(tm);
+    hi = (unsigned)__double2hiint(tm);
 }
 
 // bytes b of four packed words -> one word (byte k from word k)
@@ -72,35 +68,47 @@ __device__ __forceinline__ void transpose4(unsigned a, unsigned b, unsigned c, u
     out[2] = __builtin_amdgcn_perm(uhi, thi, 0x05040100u);
     out[3] = __builtin_amdgcn_perm(uhi, thi, 0x07060302u);
 }
-__device__ __forceinline__ unsigned pack_low_bytes(int a, int b, int c, int d) {
-    const unsigned ab = __builtin_amdgcn_perm((unsigned)b, (unsigned)a, 0x0c0c0400u);   // a0 b0 0 0
-    const unsigned cd = __builtin_amdgcn_perm((unsigned)d, (unsigned)c, 0x04000c0cu);   // 0 0 c0 d0
+__device__ __forceinline__ unsigned pack_low_bytes(unsigned a, unsigned b, unsigned c, unsigned d) {
+    const unsigned ab = __builtin_amdgcn_perm(b, a, 0x0c0c0400u);   // a0 b0 0 0
+    const unsigned cd = __builtin_amdgcn_perm(d, c, 0x04000c0cu);   // 0 0 c0 d0
     return ab | cd;
+}
+
+// five signed digit planes (plane 0 = most significant) of four values, one byte each
+__device__ __forceinline__ void planes4(const unsigned lo[4], const unsigned hi[4], unsigned pl[kI8Planes]) {
+    unsigned tr[4];
+    transpose4(lo[0], lo[1], lo[2], lo[3], tr);
+    pl[0] = pack_low_bytes(hi[0], hi[1], hi[2], hi[3]) ^ 0x80808080u;
+#pragma unroll
+    for (int p = 1; p < kI8Planes; ++p) pl[p] = tr[4 - p] ^ 0x80808080u;
 }
 
 // ---------------------------------------------------------------------------------------
 // Row-exponent table: for every (order slot, latitude pair) and super-block, eA with
-// max |z_i| < 2^(eA-1) over the super-block's steps i >= i0 (0 where none).  Same
-// arithmetic as the synthesis recurrence.
+// max |w_i| < 2^(eA-1) over the super-block's steps i >= i0 (0 where none).  The same
+// arithmetic as the synthesis kernel: frames change at every 32-step block start.
 // ---------------------------------------------------------------------------------------
-// balanced seeds (w_{i0-1}, w_{i0}) of a lane from the FP64 seeds (z_{i0-1}, z_{i0})
-__device__ __forceinline__ double2 balanced_seed(double2 sd, int i0, int nst, const int *__restrict__ dl) {
+// balanced seeds (w_{i0-1}, w_{i0}) of a lane, in the frame of the block holding step i0,
+// from the FP64 seeds (z_{i0-1}, z_{i0}); db = the order's block exponents
+__device__ __forceinline__ double2 balanced_seed(double2 sd, int i0, int nst, const int2 *__restrict__ db) {
     const int i = min(i0, nst - 1);
-    return make_double2(ldexp(sd.x, dl[max(i - 1, 0)]), ldexp(sd.y, dl[i]));
+    const int2 d = db[i / kI8KB];
+    return make_double2(ldexp(sd.x, (i & 1) ? d.x : d.y), ldexp(sd.y, (i & 1) ? d.y : d.x));
 }
 
 __global__ void i8_rowexp_kernel(const double *__restrict__ x2k, const int *__restrict__ step_ofs,
-                                 const double4 *__restrict__ abg, const int *__restrict__ delta,
+                                 const double2 *__restrict__ abk, const int2 *__restrict__ dblk,
                                  const int *__restrict__ i0tab, const double2 *__restrict__ z0tab,
                                  const int *__restrict__ sb_ofs, int mmax, int nlat_2, short *__restrict__ eA) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int slot = blockIdx.y;
     if (k >= nlat_2 || slot > mmax) return;
     const int nst = step_ofs[slot + 1] - step_ofs[slot];
-    const double4 *abm = abg + step_ofs[slot];
+    const double2 *abm = abk + (size_t)sb_ofs[slot] * kI8SB;
+    const int2 *db = dblk + (size_t)sb_ofs[slot] * kI8KBperSB;
     const double x2 = x2k[k];
     const int i0 = i0tab[(size_t)slot * nlat_2 + k];
-    const double2 sd = balanced_seed(z0tab[(size_t)slot * nlat_2 + k], i0, nst, delta + step_ofs[slot]);
+    const double2 sd = balanced_seed(z0tab[(size_t)slot * nlat_2 + k], i0, nst, db);
     short *out = eA + (size_t)sb_ofs[slot] * nlat_2 + k;
     const int nsb = sb_ofs[slot + 1] - sb_ofs[slot];
     double za = sd.x, zb = sd.y;
@@ -108,9 +116,14 @@ __global__ void i8_rowexp_kernel(const double *__restrict__ x2k, const int *__re
         double mx = 0.0;
         const int i1 = min((sb + 1) * kI8SB, nst);
         for (int i = max(sb * kI8SB, i0); i < i1; ++i) {
+            if (i > i0 && i % kI8KB == 0) {   // into the frame of block i / 32
+                const int2 dn = db[i / kI8KB], dp = db[i / kI8KB - 1];
+                za = ldexp(za, dn.y - dp.y);
+                zb = ldexp(zb, dn.x - dp.x);
+            }
             mx = fmax(mx, fabs(zb));
-            const double4 c = abm[i];
-            const double zn = fma(fma(c.x, x2, c.y), zb, c.z * za);
+            const double2 c = abm[i];
+            const double zn = fma(fma(c.x, x2, c.y), zb, za);
             za = zb;
             zb = zn;
         }
@@ -126,8 +139,8 @@ __global__ void i8_rowexp_kernel(const double *__restrict__ x2k, const int *__re
 // ---------------------------------------------------------------------------------------
 // Coefficient planes: qmix (FP64, 32 columns per 8-field group, pre-mixed as for the FP64
 // kernel) -> five digit planes per 32-step K block in the MFMA B layout and a column
-// exponent per super-block.  Block = (field group, order slot, super-block), 256 threads:
-// thread t owns column t & 31 and steps (t >> 5) + 8 s.
+// exponent per super-block.  Block = (super-block, order slot, field group), 256 threads:
+// thread t owns column t & 31 and steps (t >> 5) + 8 s of the super-block.
 //   planes[fg][kblock][p][lane][16 bytes],  lane = 32 (step & 1) + col,  byte = (step & 31) >> 1
 // ---------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void i8_slice_coeff_kernel(const double *__restrict__ qmix, int nsteps_total,
@@ -166,14 +179,14 @@ __global__ __launch_bounds__(256) void i8_slice_coeff_kernel(const double *__res
 #pragma unroll
     for (int s = 0; s < 16; ++s) {
         const int i = r0 + 8 * s;          // step within the super-block
-        unsigned p4;
-        int d4;
-        slice5(ldexp(v[s], 31 - e), p4, d4);
+        unsigned lo, hi;
+        slice40(ldexp(v[s], 39 - e), lo, hi);
         const int kb = i >> 5, sr = i & 31;
         const int lane = 32 * (sr & 1) + col, byte = sr >> 1;
+        st[kb][0][lane * 16 + byte] = (unsigned char)((hi & 0xffu) ^ 0x80u);
 #pragma unroll
-        for (int p = 0; p < 4; ++p) st[kb][p][lane * 16 + byte] = (unsigned char)(p4 >> (8 * (3 - p)));
-        st[kb][4][lane * 16 + byte] = (unsigned char)d4;
+        for (int p = 1; p < kI8Planes; ++p)
+            st[kb][p][lane * 16 + byte] = (unsigned char)(((lo >> (8 * (4 - p))) & 0xffu) ^ 0x80u);
     }
     __syncthreads();
     // 20 KB out, 16 bytes per store
@@ -185,19 +198,25 @@ __global__ __launch_bounds__(256) void i8_slice_coeff_kernel(const double *__res
 
 // ---------------------------------------------------------------------------------------
 // Synthesis kernel.  Grid: x = order x tile group (tile groups of one order on one XCD),
-// y = group of 8 NCT fields.  Block = 4 waves, each 32 latitude pairs x NCT column tiles.
-// The K blocks of a wave run as one loop; the coefficient planes of the next K block are
-// loaded while the recurrence of the current one runs.
+// y = 8-field group.  Block = 4 waves of 32 latitude pairs.  K blocks are software-pipelined:
+// the 19 MFMAs of block gb are issued between the steps of block gb + 1 (five after each
+// group of eight steps) and the coefficient planes of block gb + 1 are in flight meanwhile.
+// The block first copies the order's recurrence coefficients (a table padded with zeros to
+// whole super-blocks per order, row sb_ofs[slot] * 128 + i) to LDS; every step reads them
+// there (one broadcast address per wave).
 // ---------------------------------------------------------------------------------------
-template <int NCT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NCT == 1 ? 2 : 1, NCT == 1 ? 2 : 1))) void
-legendre_synth_i8_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) void legendre_synth_i8_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ wqk,
-    const int *__restrict__ step_ofs, const double4 *__restrict__ abg, const int *__restrict__ delta,
+    const int *__restrict__ step_ofs, const double2 *__restrict__ abk, const int2 *__restrict__ dblk,
     const int *__restrict__ i0tab, const double2 *__restrict__ z0tab, const int *__restrict__ sb_ofs, int sb_total,
     const short *__restrict__ eAtab, const unsigned char *__restrict__ planes, const int *__restrict__ eBtab,
     int lmax_spec, double2 *__restrict__ F, long four_dist, int mmax, int mres, int nlat, int nlat_2, int ltr,
     int imlim, int nfields, int lat_weight) {
+    // LDS: FP64 sums over super-blocks (one column per lane), the order's recurrence
+    // coefficients (a', b') per step and its block exponents
+    extern __shared__ double lds[];
+    double(*sums)[16][64] = reinterpret_cast<double(*)[16][64]>(lds);
+    double2 *cfl = reinterpret_cast<double2 *>(lds + kI8Waves * 16 * 64);
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int ntiles = (nlat_2 + kI8Lat - 1) / kI8Lat;
@@ -205,29 +224,34 @@ legendre_synth_i8_kernel(
     const int bx = xcd_cluster(blockIdx.x, gridDim.x, ntg);
     const int im = bx / ntg;
     const int tile = (bx - im * ntg) * kI8Waves + wave;
-    if (tile >= ntiles) return;   // no block-level synchronization in this kernel
-    const int fg0 = NCT * blockIdx.y;               // first 8-field group of this wave
-    const int kr = tile * kI8Lat + (lane & 31);     // this lane's latitude pair
-    const bool kvalid = kr < nlat_2;
-    const int kk = kvalid ? kr : nlat_2 - 1;
-    const int par = lane >> 5;
-
-    double out[NCT][16];
-#pragma unroll
-    for (int c = 0; c < NCT; ++c)
-#pragma unroll
-        for (int j = 0; j < 16; ++j) out[c][j] = 0.0;
-
     const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, 0);
     const int nsteps = (im <= imlim) ? o.nsteps : 0;
+    const int gb_end = (nsteps + kI8KB - 1) / kI8KB;
+    int2 *dbl = reinterpret_cast<int2 *>(cfl + gb_end * kI8KB);
     if (nsteps > 0) {
-        const int sofs = step_ofs[o.slot];
-        const __attribute__((address_space(4))) double *abm =
-            (const __attribute__((address_space(4))) double *)(abg + sofs);   // uniform: scalar loads
+        const int sbofs = sb_ofs[o.slot];
+        const double2 *src = abk + (size_t)sbofs * kI8SB;   // zero past the order's end
+        for (int i = threadIdx.x; i < gb_end * kI8KB; i += blockDim.x) cfl[i] = src[i];
+        for (int i = threadIdx.x; i < gb_end; i += blockDim.x) dbl[i] = dblk[(size_t)sbofs * kI8KBperSB + i];
+    }
+    __syncthreads();
+    if (tile >= ntiles) return;   // the only block-level synchronization is above
+    const int fg = blockIdx.y;
+    const int kr = tile * kI8Lat + (lane & 31);   // this lane's latitude pair
+    const bool kvalid = kr < nlat_2;
+    const int kk = kvalid ? kr : nlat_2 - 1;
+    const int half = lane >> 5;
+    double *outl = &sums[wave][0][lane];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) outl[64 * j] = 0.0;
+
+    if (nsteps > 0) {
         const double x2 = x2k[kk];
         int i0 = kvalid ? i0tab[(size_t)o.slot * nlat_2 + kk] : nsteps;
         if (i0 > nsteps) i0 = nsteps;
-        const double2 sd = balanced_seed(z0tab[(size_t)o.slot * nlat_2 + kk], i0, nsteps, delta + sofs);
+        const int sbofs = sb_ofs[o.slot];
+        const double2 sd = balanced_seed(z0tab[(size_t)o.slot * nlat_2 + kk], i0, nsteps,
+                                         dblk + (size_t)sbofs * kI8KBperSB);
         int wmin = i0, wmax = i0;
         for (int off = 32; off >= 1; off >>= 1) {
             wmin = min(wmin, __shfl_xor(wmin, off));
@@ -235,105 +259,135 @@ legendre_synth_i8_kernel(
         }
         wmin = __builtin_amdgcn_readfirstlane(wmin);
         wmax = __builtin_amdgcn_readfirstlane(wmax);
-        const int sbofs = sb_ofs[o.slot];
         const short *eArow = eAtab + (size_t)sbofs * nlat_2 + kk;
-        const unsigned char *pl0 = planes + ((size_t)fg0 * 4 * sb_total + 4 * (size_t)sbofs) * kI8Planes * 1024 +
+        const unsigned char *pl0 = planes + ((size_t)fg * 4 * sb_total + 4 * (size_t)sbofs) * kI8Planes * 1024 +
                                    (size_t)lane * 16;
-        const size_t fg_stride = (size_t)4 * sb_total * kI8Planes * 1024;
-        const int *eB0 = eBtab + ((size_t)fg0 * sb_total + sbofs) * 32 + (lane & 31);
-        auto load_b = [&](int gb, i32x4 (&B)[NCT][kI8Planes]) {
-#pragma unroll
-            for (int c = 0; c < NCT; ++c)
+        const int *eB0 = eBtab + ((size_t)fg * sb_total + sbofs) * 32 + (lane & 31);
+        const int gb_first = wmin / kI8KB, gb_end = (nsteps + kI8KB - 1) / kI8KB;
+        if (gb_first < gb_end) {
+            double za = 0.0, zb = 0.0;
+            int e = 39;                    // state scale: values are w * 2^(39 - e)
+            const int nsb = (gb_end + kI8KBperSB - 1) / kI8KBperSB;
+            int e_sb = -1;                 // super-block of e; e_next = scale of the following one
+            int e_next = kvalid ? (int)eArow[(size_t)(gb_first / kI8KBperSB) * nlat_2] : 39;
+            i32x16 acc[kI8ND];
+            i32x4 A[kI8Planes], Bc[kI8Planes];
+            auto load_b = [&](int gb, i32x4 (&B)[kI8Planes]) {
 #pragma unroll
                 for (int q = 0; q < kI8Planes; ++q)
-                    B[c][q] = *reinterpret_cast<const i32x4 *>(pl0 + c * fg_stride +
-                                                               ((size_t)gb * kI8Planes + q) * 1024);
-        };
-        double za = 0.0, zb = 0.0;
-        int e = 31, eprev = 31;            // state scale: values are z * 2^(31 - e)
-        double seedscale = 1.0;
-        const int gb_first = wmin / kI8KB, gb_end = (nsteps + kI8KB - 1) / kI8KB;
-        i32x16 acc[NCT][kI8ND];
-        i32x4 Bc[NCT][kI8Planes], Bn[NCT][kI8Planes];
-        load_b(gb_first, Bc);
-        for (int gb = gb_first; gb < gb_end; ++gb) {
-            const int sb = gb / kI8KBperSB;
-            if (gb == gb_first || gb % kI8KBperSB == 0) {   // a new super-block: scale, zero sums
-                e = kvalid ? (int)eArow[(size_t)sb * nlat_2] : 0;
-                const int sh = eprev - e;
-                za = ldexp(za, sh);
-                zb = ldexp(zb, sh);
-                eprev = e;
-                seedscale = ldexp(1.0, 31 - e);
-#pragma unroll
-                for (int c = 0; c < NCT; ++c)
-#pragma unroll
-                    for (int d = 0; d < kI8ND; ++d) acc[c][d] = (i32x16){0};
-            }
-            if (gb + 1 < gb_end) load_b(gb + 1, Bn);   // in flight during the recurrence
-            const int ibase = gb * kI8KB;
-            // recurrence over the block's 32 steps; this lane keeps the steps of its parity
-            double keep[16];
-            auto steps = [&](auto inject) {
-#pragma unroll
-                for (int s = 0; s < kI8KB; ++s) {
-                    const int i = ibase + s;
-                    if (decltype(inject)::value && i == i0) {
-                        za = sd.x * seedscale;
-                        zb = sd.y * seedscale;
-                    }
-                    if ((s & 1) == 0) keep[s >> 1] = zb;
-                    else keep[s >> 1] = par ? zb : keep[s >> 1];
-                    const int ic = 4 * min(i, nsteps - 1);
-                    const double zn = fma(fma(abm[ic], x2, abm[ic + 1]), zb, abm[ic + 2] * za);
-                    za = zb;
-                    zb = zn;
-                }
+                    B[q] = *reinterpret_cast<const i32x4 *>(pl0 + ((size_t)gb * kI8Planes + q) * 1024);
             };
-            if (ibase <= wmax) steps(std::true_type{});   // some lane of the wave starts here
-            else steps(std::false_type{});
-            // digit planes of the kept values: A[p] = 16 bytes
-            i32x4 A[kI8Planes];
+            // MFMA k of a K block (k = 0..18): coefficient plane q, value plane p, p + q <= 5;
+            // consecutive k use different accumulators
+            auto mfma_k = [&](auto kc) {
+                constexpr int k = decltype(kc)::value;
+                constexpr int q = k < 5 ? 0 : k < 10 ? 1 : k < 14 ? 2 : k < 17 ? 3 : 4;
+                constexpr int p = k - (q == 0 ? 0 : q == 1 ? 5 : q == 2 ? 10 : q == 3 ? 14 : 17);
+                acc[p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[p], Bc[q], acc[p + q], 0, 0, 0);
+            };
+            constexpr int kMfma = 19;
+            auto mfma_range = [&](auto k0c) {
+                constexpr int k0 = decltype(k0c)::value;
+                if constexpr (k0 + 0 < kMfma) mfma_k(std::integral_constant<int, k0 + 0>{});
+                if constexpr (k0 + 1 < kMfma) mfma_k(std::integral_constant<int, k0 + 1>{});
+                if constexpr (k0 + 2 < kMfma) mfma_k(std::integral_constant<int, k0 + 2>{});
+                if constexpr (k0 + 3 < kMfma) mfma_k(std::integral_constant<int, k0 + 3>{});
+                if constexpr (k0 + 4 < kMfma) mfma_k(std::integral_constant<int, k0 + 4>{});
+            };
+            // recurrence values of block gb -> A (and the super-block rescale).  Lane half h
+            // keeps the values of steps 2j + h (byte j).  With WITH_MFMA the MFMAs of the previous
+            // block (operands A, Bc) are issued in between, five after each group of eight steps.
+            auto gen = [&](int gb, auto inject, auto with_mfma) {
+                int enew = e;
+                if (gb / kI8KBperSB != e_sb) {   // a new super-block: its scale was loaded ahead
+                    e_sb = gb / kI8KBperSB;
+                    enew = e_next;
+                    e_next = kvalid ? (int)eArow[(size_t)min(e_sb + 1, nsb - 1) * nlat_2] : e_next;
+                }
+                const int sh = e - enew;
+                const int2 dn = dbl[gb], dp = dbl[max(gb - 1, 0)];   // into the frame of block gb
+                za = ldexp(za, sh + dn.y - dp.y);
+                zb = ldexp(zb, sh + dn.x - dp.x);
+                e = enew;
+                const double seedscale = ldexp(1.0, 39 - e);
+                const int ibase = gb * kI8KB;
+                const double2 *cb = cfl + ibase;
+                unsigned An[kI8Planes][4];
+                auto group = [&](auto gc) {
+                    constexpr int g = decltype(gc)::value;
+                    unsigned lo[4], hi[4];
+                    double v = 0.0;
+                    double2 cg[8];   // the group's coefficients, requested together
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                unsigned p4[4];
-                int d4[4];
+                    for (int u = 0; u < 8; ++u) cg[u] = cb[8 * g + u];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) slice5(keep[4 * g + u], p4[u], d4[u]);
-                unsigned tr[4];
-                transpose4(p4[0], p4[1], p4[2], p4[3], tr);
+                    for (int u = 0; u < 8; ++u) {
+                        const int s = 8 * g + u;
+                        if (decltype(inject)::value && ibase + s == i0) {
+                            za = sd.x * seedscale;
+                            zb = sd.y * seedscale;
+                        }
+                        if (u & 1) {
+                            v = half ? zb : v;
+                            slice40(v, lo[u >> 1], hi[u >> 1]);
+                        } else {
+                            v = zb;
+                        }
+                        const double2 c = cg[u];
+                        const double zn = fma(fma(c.x, x2, c.y), zb, za);
+                        za = zb;
+                        zb = zn;
+                    }
+                    unsigned pl[kI8Planes];
+                    planes4(lo, hi, pl);
 #pragma unroll
-                for (int p = 0; p < 4; ++p) A[p][g] = (int)tr[3 - p];   // plane 0 = top digits
-                A[4][g] = (int)pack_low_bytes(d4[0], d4[1], d4[2], d4[3]);
-            }
-#pragma unroll
-            for (int c = 0; c < NCT; ++c)
+                    for (int p = 0; p < kI8Planes; ++p) An[p][g] = pl[p];
+                    if constexpr (decltype(with_mfma)::value) mfma_range(std::integral_constant<int, 5 * g>{});
+                    __builtin_amdgcn_sched_barrier(0);   // bound the live range of hoisted loads
+                };
+                group(std::integral_constant<int, 0>{});
+                group(std::integral_constant<int, 1>{});
+                group(std::integral_constant<int, 2>{});
+                group(std::integral_constant<int, 3>{});
 #pragma unroll
                 for (int p = 0; p < kI8Planes; ++p)
 #pragma unroll
-                    for (int q = 0; q < kI8Planes; ++q)
-                        if (p + q < kI8ND)
-                            acc[c][p + q] = __builtin_amdgcn_mfma_i32_32x32x32_i8(A[p], Bc[c][q], acc[c][p + q], 0, 0, 0);
-            if (gb % kI8KBperSB == kI8KBperSB - 1 || gb + 1 == gb_end) {
-                // super-block result into FP64: D row (register j) = (j & 3) + 8 (j >> 2) + 4 (lane >> 5),
-                // column = lane & 31
+                    for (int w = 0; w < 4; ++w) A[p][w] = (int)An[p][w];
+            };
+            if (gb_first * kI8KB <= wmax) gen(gb_first, std::true_type{}, std::false_type{});
+            else gen(gb_first, std::false_type{}, std::false_type{});
+            load_b(gb_first, Bc);
+            int gb = gb_first;
+            for (int sb = gb_first / kI8KBperSB; sb * kI8KBperSB < gb_end; ++sb) {
+                const int emf = e;   // scale of this super-block's values
 #pragma unroll
-                for (int c = 0; c < NCT; ++c) {
-                    const int eb = eB0[(size_t)c * sb_total * 32 + (size_t)sb * 32];
-#pragma unroll
-                    for (int j = 0; j < 16; ++j) {
-                        const int er = __shfl(e, (j & 3) + 8 * (j >> 2) + 4 * par);
-                        double t = (double)acc[c][kI8ND - 1][j];
-#pragma unroll
-                        for (int d = kI8ND - 2; d >= 0; --d) t = fma(t, 0x1p-8, (double)acc[c][d][j]);
-                        out[c][j] += ldexp(t, er + eb - 14);
+                for (int d = 0; d < kI8ND; ++d) acc[d] = (i32x16){0};
+                const int gend = min((sb + 1) * kI8KBperSB, gb_end);
+                for (; gb < gend; ++gb) {
+                    // the other wave on the SIMD runs its VALU work while these MFMAs execute
+                    const bool more = gb + 1 < gb_end;
+                    mfma_range(std::integral_constant<int, 0>{});
+                    mfma_range(std::integral_constant<int, 5>{});
+                    mfma_range(std::integral_constant<int, 10>{});
+                    mfma_range(std::integral_constant<int, 15>{});
+                    if (more) {
+                        load_b(gb + 1, Bc);   // in flight during the next block's recurrence
+                        if ((gb + 1) * kI8KB <= wmax) gen(gb + 1, std::true_type{}, std::false_type{});
+                        else gen(gb + 1, std::false_type{}, std::false_type{});
                     }
                 }
+                // super-block result into FP64: D row (register j) = (j & 3) + 8 (j >> 2) + 4 half,
+                // column = lane & 31
+                const int eb = eB0[(size_t)sb * 32];
+#pragma unroll
+                for (int j = 0; j < 16; ++j) {
+                    const int er = __shfl(emf, (j & 3) + 8 * (j >> 2) + 4 * half);
+                    double t = (double)acc[kI8ND - 1][j];
+#pragma unroll
+                    for (int d = kI8ND - 2; d >= 0; --d) t = fma(t, 0x1p-8, (double)acc[d][j]);
+                    outl[64 * j] += ldexp(t, er + eb - 14);
+                }
             }
-#pragma unroll
-            for (int c = 0; c < NCT; ++c)
-#pragma unroll
-                for (int q = 0; q < kI8Planes; ++q) Bc[c][q] = Bn[c][q];
         }
     }
 
@@ -342,24 +396,22 @@ legendre_synth_i8_kernel(
     const int reim = col & 1;
     const bool o_lane = (col & 2) != 0;
     const bool zero_im = (im == 0) && reim;
+    const int f = 8 * fg + (col >> 2);
 #pragma unroll
     for (int j = 0; j < 16; ++j) {
-        const int krow = tile * kI8Lat + (j & 3) + 8 * (j >> 2) + 4 * par;
+        const int krow = tile * kI8Lat + (j & 3) + 8 * (j >> 2) + 4 * half;
         const int kc = min(krow, nlat_2 - 1);
         const double x = xk[kc];
         const double xs = o_lane ? -x : x;
         const double wz = zero_im ? 0.0 : (lat_weight ? wqk[kc] * ((2 * krow + 1 == nlat) ? 2.0 : 1.0) : 1.0);
         const int row = o_lane ? (nlat - 1 - krow) : krow;
-#pragma unroll
-        for (int c = 0; c < NCT; ++c) {
-            const double ov = quad_perm<0xEE>(out[c][j]);   // lanes (2, 3, 2, 3)
-            const double ev = quad_perm<0x44>(out[c][j]);   // lanes (0, 1, 0, 1)
-            const double val = fma(xs, ov, ev) * wz;
-            const int f = 8 * (fg0 + c) + (col >> 2);
-            if (krow < nlat_2 && f < nfields) {
-                double *Ff = reinterpret_cast<double *>(F + (size_t)f * four_dist + (size_t)im * nlat) + reim;
-                Ff[2 * row] = val;
-            }
+        const double sum = outl[64 * j];
+        const double ov = quad_perm<0xEE>(sum);   // lanes (2, 3, 2, 3)
+        const double ev = quad_perm<0x44>(sum);   // lanes (0, 1, 0, 1)
+        const double val = fma(xs, ov, ev) * wz;
+        if (krow < nlat_2 && f < nfields) {
+            double *Ff = reinterpret_cast<double *>(F + (size_t)f * four_dist + (size_t)im * nlat) + reim;
+            Ff[2 * row] = val;
         }
     }
 }
@@ -369,8 +421,12 @@ legendre_synth_i8_kernel(
 // ---------------------------------------------------------------------------------------
 void launch_i8_rowexp(const DeviceTables *t, int mmax, int nlat_2, hipStream_t s) {
     dim3 grid((nlat_2 + 63) / 64, mmax + 1);
-    hipLaunchKernelGGL(i8_rowexp_kernel, grid, dim3(64), 0, s, t->x2, t->step_ofs, t->abg, t->delta, t->i0, t->z0,
+    hipLaunchKernelGGL(i8_rowexp_kernel, grid, dim3(64), 0, s, t->x2, t->step_ofs, t->abk, t->dblk, t->i0, t->z0,
                        t->sb_ofs, mmax, nlat_2, t->eA);
+}
+
+size_t i8_synth_lds_bytes(int sb_max) {
+    return sizeof(double) * kI8Waves * 16 * 64 + (size_t)sb_max * (kI8SB * sizeof(double2) + kI8KBperSB * sizeof(int2));
 }
 
 size_t i8_planes_bytes(int nfields, int sb_total) {
@@ -392,11 +448,11 @@ void launch_legendre_synth_i8(const LegendreArgs &a, int spec_ltop, double2 *F, 
                        t->step_ofs, t->delta, t->sb_ofs, t->sb_total, nfg_valid, planes, eB);
     const int ntiles = (a.nlat_2 + kI8Lat - 1) / kI8Lat;
     const int ntg = (ntiles + kI8Waves - 1) / kI8Waves;
-    // one 8-field column tile per wave keeps the kernel within 256 registers (2 waves per SIMD);
-    // two tiles per wave spill
-    dim3 grid((a.mmax + 1) * ntg, nfg);
-    hipLaunchKernelGGL(legendre_synth_i8_kernel<1>, grid, dim3(64 * kI8Waves), 0, s, t->x, t->x2, t->wq, t->step_ofs,
-                       t->abg, t->delta, t->i0, t->z0, t->sb_ofs, t->sb_total, t->eA, planes, eB, spec_ltop, F, four_dist,
+    dim3 grid((a.mmax + 1) * ntg, nfg_valid);
+    const size_t lds = i8_synth_lds_bytes(t->sb_max);
+    ensure_lds_limit(reinterpret_cast<const void *>(legendre_synth_i8_kernel), lds);
+    hipLaunchKernelGGL(legendre_synth_i8_kernel, grid, dim3(64 * kI8Waves), lds, s, t->x, t->x2, t->wq, t->step_ofs,
+                       t->abk, t->dblk, t->i0, t->z0, t->sb_ofs, t->sb_total, t->eA, planes, eB, spec_ltop, F, four_dist,
                        a.mmax, a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.lat_weight);
 }
 
