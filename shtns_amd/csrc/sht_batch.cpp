@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
+#include <string>
 #include <vector>
 
 #include "sht_internal.h"
@@ -51,7 +53,63 @@ bool tune_verbose() {
     return v;
 }
 
+// ---- tuning file (the GPU counterpart of SHTns's SHT_LOAD_SAVE_CFG, sht_init.c:975-1061) ----
+// One line per tuned (configuration, batch size), appended by cushtns_tune:
+//   shtns_amd-tune 1 <gfx arch> lmax mmax mres nlat nphi grid_flags nfields  ncg nw ancg fsyn fana
+// The file is $SHTNS_AMD_CFG, else "shtns_amd_cfg" in the working directory.  Lines that do
+// not parse, or are for another device or build, are skipped.
+const char *tune_file() {
+    const char *e = getenv("SHTNS_AMD_CFG");
+    return (e && *e) ? e : "shtns_amd_cfg";
+}
+
+std::string device_arch(int dev) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown";
+    std::string a = p.gcnArchName;
+    const size_t colon = a.find(':');   // "gfx950:sramecc+:xnack-" -> "gfx950"
+    return colon == std::string::npos ? a : a.substr(0, colon);
+}
+
+bool tune_cache_save(shtns_cfg c, int nfields) {
+    FILE *f = fopen(tune_file(), "a");
+    if (!f) return false;
+    fprintf(f, "shtns_amd-tune 1 %s %d %d %d %u %u %d %d  %d %d %d %d %d\n", device_arch(c->device).c_str(), c->lmax,
+            c->mmax, c->mres, c->nlat, c->nphi, c->grid_flags, nfields, c->tune_synth_ncg, c->tune_anal_nw,
+            c->tune_anal_ncg, c->tune_fft_synth, c->tune_fft_anal);
+    const bool ok = fclose(f) == 0;
+    return ok;
+}
+
 }  // namespace
+
+bool shtns_amd::tune_cache_apply(shtns_cfg c, int nfields) {
+    FILE *f = fopen(tune_file(), "r");
+    if (!f) return false;
+    const std::string arch = device_arch(c->device);
+    char line[512], tag[32], a[64];
+    int found = 0, ch[5] = {0, 0, 0, 0, 0};
+    while (fgets(line, sizeof line, f)) {
+        int ver, lmax, mmax, mres, gflags, nf, v[5];
+        unsigned nlat, nphi;
+        if (sscanf(line, "%31s %d %63s %d %d %d %u %u %d %d %d %d %d %d %d", tag, &ver, a, &lmax, &mmax, &mres, &nlat,
+                   &nphi, &gflags, &nf, &v[0], &v[1], &v[2], &v[3], &v[4]) != 15)
+            continue;
+        if (strcmp(tag, "shtns_amd-tune") != 0 || ver != 1 || arch != a) continue;
+        if (lmax != c->lmax || mmax != c->mmax || mres != c->mres || nlat != c->nlat || nphi != c->nphi ||
+            gflags != c->grid_flags)
+            continue;
+        if (nfields > 0 && nf != nfields) continue;
+        std::copy(v, v + 5, ch);   // the last matching line wins
+        found = 1;
+    }
+    fclose(f);
+    if (!found || cushtns_set_tuning(c, ch) != 0) {
+        clear_error();
+        return false;
+    }
+    return true;
+}
 
 extern "C" int cushtns_tune(shtns_cfg c, int nfields) {
     clear_error();
@@ -64,6 +122,7 @@ extern "C" int cushtns_tune(shtns_cfg c, int nfields) {
         return -1;
     }
     (void)hipSetDevice(c->device);
+    if (c->load_save && tune_cache_apply(c, nfields)) return 0;   // tuned by an earlier run
     const hipStream_t s = c->comp_stream;
     DevBuf q, v;
     const size_t qb = (size_t)nfields * c->nlm * sizeof(cplx), vb = (size_t)nfields * c->nspat * sizeof(double);
@@ -131,6 +190,8 @@ extern "C" int cushtns_tune(shtns_cfg c, int nfields) {
         return -1;
     }
     clear_error();
+    if (c->load_save && !tune_cache_save(c, nfields))
+        fprintf(stderr, "! Warning ! shtns_amd could not save the tuning to %s\n", tune_file());
     return 0;
 }
 

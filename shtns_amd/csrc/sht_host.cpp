@@ -537,6 +537,8 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
     c->layout = (layout & SHT_PHI_CONTIGUOUS) ? SHT_PHI_CONTIGUOUS : SHT_THETA_CONTIGUOUS;
     if (layout & SHT_SOUTH_POLE_FIRST) c->layout |= SHT_SOUTH_POLE_FIRST;
     c->eps = eps;
+    c->load_save = (layout & SHT_LOAD_SAVE_CFG) != 0;
+    c->grid_flags = (int)flags & ~SHT_LOAD_SAVE_CFG;
     c->ct = new double[2 * (size_t)c->nlat];
     c->st = c->ct + c->nlat;
     std::vector<double> w(c->nlat);
@@ -566,6 +568,7 @@ extern "C" int shtns_set_grid_auto(shtns_cfg c, enum shtns_type flags, double ep
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) == hipSuccess && ndev > 0) {
             if (device_init(c) < 0) fprintf(stderr, "*** [shtns_amd] GPU initialization failed: %s\n", g_last_msg);
+            else if (c->load_save) (void)tune_cache_apply(c, 0);   // shapes saved by an earlier run
         } else {
             (void)hipGetLastError();
         }

@@ -116,9 +116,17 @@ struct shtns_info {
     size_t xscratch_bytes = 0;
     // kernel shapes picked by cushtns_tune (0: automatic)
     int tune_synth_ncg = 0, tune_anal_nw = 0, tune_anal_ncg = 0, tune_fft_synth = 0, tune_fft_anal = 0;
+    // SHT_LOAD_SAVE_CFG given to shtns_set_grid: kernel shapes are read from / appended to
+    // the tuning file; grid_flags = the other flags of that call (part of the file's key)
+    bool load_save = false;
+    int grid_flags = 0;
 };
 
 namespace shtns_amd {
+
+// tuning file (sht_batch.cpp): apply the entry saved for this configuration and batch size
+// (nfields <= 0: the last entry for this configuration, any batch size); true if one was found
+bool tune_cache_apply(shtns_cfg c, int nfields);
 
 // kernel launchers (sht_kernels.hip)
 struct LegendreArgs {

@@ -116,3 +116,13 @@ def test_sizes_validation_and_errors_without_gpu():
         with pytest.raises(RuntimeError, match="no usable GPU"):
             s.synth(np.zeros(s.nlm, dtype=complex))
         assert _lib.load().shtns_last_error() != 0
+
+
+def test_load_save_flag_accepted_without_gpu(tmp_path, monkeypatch):
+    """SHT_LOAD_SAVE_CFG is accepted by shtns_set_grid; with no GPU nothing is read or written."""
+    monkeypatch.setenv("SHTNS_AMD_CFG", str(tmp_path / "cfg"))
+    s = shtns_amd.sht(31)
+    s.set_grid(48, 96, flags=shtns_amd.SHT_THETA_CONTIGUOUS | shtns_amd.sht_gauss | shtns_amd.SHT_LOAD_SAVE_CFG,
+               polar_opt=0.0)
+    assert s.nlat == 48 and s.nphi == 96
+    assert not (tmp_path / "cfg").exists()
