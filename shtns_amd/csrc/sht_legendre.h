@@ -143,7 +143,13 @@ struct Contract {
 // ---------------------------------------------------------------------------------------
 constexpr int kSynthWaves = 4;
 
-template <int NCG, int PREC>
+// NF4 > 0 (FP64, NCG = 1, at most 3 fields): field c of the group is contracted with
+// v_mfma_f64_4x4x4f64 (four 4x4x4 blocks: 16 latitudes x 4 steps x the field's 4 columns),
+// so no matrix-core columns are spent on absent fields.  gfx950 lane layouts of that
+// instruction: A lane l = (block (l>>2)&3, row l&3, k l>>4), B lane l = (block (l>>2)&3,
+// k l>>4, col l&3), D lane l = (block (l>>2)&3, row l>>4, col l&3); with block b and row i
+// taken as latitude 4b + i of a 16-latitude row tile, the A operand is the 16x16x4 one.
+template <int NCG, int PREC, int NF4 = 0>
 __global__ __launch_bounds__(256) void legendre_synth_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
@@ -169,15 +175,18 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     const int kk = kvalid ? k : (nlat_2 - 1);
 
     using C = Contract<PREC>;
-    double4_t acc[4][NCG];   // FP64 sums
+    static_assert(NF4 == 0 || (PREC == 0 && NCG == 1 && NF4 <= 3), "4x4x4 path: FP64, one group of <= 3 fields");
+    constexpr int NA = NF4 ? NF4 : NCG;   // accumulator sets: fields (4x4x4) or column groups
+    using acc64_t = std::conditional_t<NF4 == 0, double4_t, double>;
+    acc64_t acc[4][NA];   // FP64 sums
     typename C::acc_t part[4][NCG];   // FP32 partial sums over 16 steps (PREC > 0; unused at PREC 0)
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
+    for (int r = 0; r < 4; ++r) {
 #pragma unroll
-        for (int c = 0; c < NCG; ++c) {
-            acc[r][c] = (double4_t){0.0, 0.0, 0.0, 0.0};
-            part[r][c] = C::zero();
-        }
+        for (int c = 0; c < NA; ++c) acc[r][c] = acc64_t{};
+#pragma unroll
+        for (int c = 0; c < NCG; ++c) part[r][c] = C::zero();
+    }
     auto flush = [&]() {
         if constexpr (PREC > 0) {
 #pragma unroll
@@ -244,11 +253,16 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
         };
         auto load_b = [&](int i, double *b) {
             const int ii = min(i + (lane >> 4), slot_rows - 1);   // rows >= nsteps hold zeros
+            if constexpr (NF4 > 0) {
 #pragma unroll
-            for (int c = 0; c < NCG; ++c) b[c] = qb[(size_t)ii * NCOL + c * 16 + (lane & 15)];
+                for (int c = 0; c < NA; ++c) b[c] = qb[(size_t)ii * NCOL + c * 4 + (lane & 3)];
+            } else {
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) b[c] = qb[(size_t)ii * NCOL + c * 16 + (lane & 15)];
+            }
         };
 
-        double bcur[NCG], bnxt[NCG];
+        double bcur[NA], bnxt[NA];
         advance(ibeg, 0);
         load_b(ibeg, bcur);
         int buf = 0, blk = 0;
@@ -259,19 +273,27 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
             typename C::a_t a[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = C::load_z(&ytile[wave][buf][lane >> 4][r * 16 + (lane & 15)]);
+            if constexpr (NF4 > 0) {
 #pragma unroll
-            for (int c = 0; c < NCG; ++c) {
-                const typename C::b_t bb = C::split(bcur[c]);
+                for (int c = 0; c < NA; ++c)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    if constexpr (PREC == 0) acc[r][c] = C::mma(a[r], bb, acc[r][c]);
-                    else part[r][c] = C::mma(a[r], bb, part[r][c]);
+                    for (int r = 0; r < 4; ++r)
+                        acc[r][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[r], bcur[c], acc[r][c], 0, 0, 0);
+            } else {
+#pragma unroll
+                for (int c = 0; c < NCG; ++c) {
+                    const typename C::b_t bb = C::split(bcur[c]);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        if constexpr (PREC == 0) acc[r][c] = C::mma(a[r], bb, acc[r][c]);
+                        else part[r][c] = C::mma(a[r], bb, part[r][c]);
+                    }
                 }
             }
             if (i + 4 < nsteps) advance(i + 4, buf ^ 1);
             buf ^= 1;
 #pragma unroll
-            for (int c = 0; c < NCG; ++c) bcur[c] = bnxt[c];
+            for (int c = 0; c < NA; ++c) bcur[c] = bnxt[c];
             if ((blk & 3) == 3) flush();
         }
         flush();
@@ -281,10 +303,32 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     // row tile r (latitudes tile*64 + 16r + row).  Columns are 4 fields x (E.re, E.im, O.re,
     // O.im), so a quad of lanes is one field's (E, O) pairs: every lane reads its pair's O and
     // E values (DPP), E lanes write north = E + x O and O lanes south = E - x O.
-    const int col = lane & 15;
+    const int col = NF4 ? (lane & 3) : (lane & 15);
     const int reim = col & 1;
     const bool o_lane = (col & 2) != 0;
     const bool zero_im = (im == 0) && reim;   // order 0 has no imaginary part
+    if constexpr (NF4 > 0) {
+        // D lane l: latitude tile*64 + 16 r + 4 ((l >> 2) & 3) + (l >> 4), column l & 3
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int kr = tile * 64 + r * 16 + 4 * ((lane >> 2) & 3) + (lane >> 4);
+            const int kc = min(kr, nlat_2 - 1);
+            const double x = xk[kc];
+            const double xs = o_lane ? -x : x;
+            const double wz = zero_im ? 0.0 : (lat_weight ? wqk[kc] * ((2 * kr + 1 == nlat) ? 2.0 : 1.0) : 1.0);
+            const int row = o_lane ? (nlat - 1 - kr) : kr;
+#pragma unroll
+            for (int c = 0; c < NA; ++c) {
+                const int f = fbase + c;
+                const double o = quad_perm<0xEE>(acc[r][c]);
+                const double e = quad_perm<0x44>(acc[r][c]);
+                const double val = fma(xs, o, e) * wz;
+                if (kr < nlat_2 && f < nfields)
+                    reinterpret_cast<double *>(F + (size_t)f * four_dist + (size_t)im * nlat)[2 * row + reim] = val;
+            }
+        }
+        return;
+    } else {
     // every latitude of the tile and every field of the group exist (wave-uniform)
     const bool full = (tile * 64 + 64 <= nlat_2) && (fbase + 4 * NCG <= nfields);
     auto store_tile = [&](auto check) {
@@ -321,6 +365,13 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
     };
     if (full) store_tile(std::false_type{});
     else store_tile(std::true_type{});
+    }
+}
+
+// SHTNS_AMD_SYNTH_16=1: keep the 16x16x4 kernels for 1 to 3 fields (for comparisons)
+inline bool synth_force16() {
+    static const bool f = getenv("SHTNS_AMD_SYNTH_16") != nullptr;
+    return f;
 }
 
 inline int synth_ncg(int nfields, int want = 0) {
@@ -336,11 +387,19 @@ void synth_tiles_launch(const LegendreArgs &a, int spec_ltop, double2 *F, long f
     const int ncg = synth_ncg(a.nfields, a.synth_ncg);
     const int ng = (a.nfields + 4 * ncg - 1) / (4 * ncg);
     dim3 grid((a.mmax + 1) * ntg, ng);
-#define SYNTH_LAUNCH(NC)                                                                                         \
-    hipLaunchKernelGGL((legendre_synth_kernel<NC, PREC>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq, \
+#define SYNTH_LAUNCH(NC, ...)                                                                                    \
+    hipLaunchKernelGGL((legendre_synth_kernel<NC, PREC, ##__VA_ARGS__>), grid, dim3(64 * kSynthWaves), 0, s, t->x, t->x2, t->st, t->wq, \
                        t->step_ofs, t->ab, a.vector_mode ? t->i0v : t->i0, a.vector_mode ? t->z0v : t->z0, qmix, \
                        nsteps_total, spec_ltop, F, four_dist, a.mmax,                                              \
                        a.mres, a.nlat, a.nlat_2, a.ltr, a.imlim, a.nfields, a.vector_mode, a.lat_weight, tile0, ntl)
+    if constexpr (PREC == 0) {   // 1 to 3 fields: 4x4x4 blocks, no idle matrix-core columns
+        if (a.nfields <= 3 && ncg == 1 && !synth_force16()) {
+            if (a.nfields == 1) SYNTH_LAUNCH(1, 1);
+            else if (a.nfields == 2) SYNTH_LAUNCH(1, 2);
+            else SYNTH_LAUNCH(1, 3);
+            return;
+        }
+    }
     if (ncg == 4) SYNTH_LAUNCH(4);
     else if (ncg == 2) SYNTH_LAUNCH(2);
     else SYNTH_LAUNCH(1);
@@ -359,7 +418,11 @@ void synth_tiles_launch(const LegendreArgs &a, int spec_ltop, double2 *F, long f
 // the waves and applies the post-mix (even: ue_i acc_i + ue1_{i-1} acc_{i-1}; odd:
 // uo_i acc_i) and stores Q.  Fixed summation order: results are deterministic.
 // ---------------------------------------------------------------------------------------
-template <int NW, int NLR, int NCG, int PREC>
+// NF4 > 0 (FP64, NCG = 1, at most 3 fields): field c is contracted with v_mfma_f64_4x4x4f64
+// (four 4x4x4 blocks = 16 steps x 4 latitudes x the field's 4 columns; the A operand is the
+// 16x16x4 one, B lane l holds column l & 3 of latitude 4 kg + (l >> 4) in every block, and
+// D lane l holds step 4 ((l >> 2) & 3) + (l >> 4), column l & 3).
+template <int NW, int NLR, int NCG, int PREC, int NF4 = 0>
 __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
@@ -367,7 +430,9 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const double2 *__restrict__ F, long four_dist, double2 *__restrict__ Q, long spec_dist, int lmax_spec,
     int spec_ltop, int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
     double mpos_scale, int lat_weight, int cf_lds) {
-    constexpr int NCOL = 16 * NCG;
+    static_assert(NF4 == 0 || (PREC == 0 && NCG == 1 && NF4 <= 3), "4x4x4 path: FP64, one group of <= 3 fields");
+    constexpr int NA = NF4 ? NF4 : NCG;           // operand / accumulator sets
+    constexpr int NCOL = NF4 ? 4 * NF4 : 16 * NCG;
     __shared__ double ytile[NW][16][65];
     // cf_lds: the order's recurrence coefficients (alpha_i, beta_i), i < nsteps, are copied to
     // LDS once per block (dynamic LDS), so each 16-step group reads them at LDS latency instead
@@ -409,7 +474,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     double za[NLR], zb[NLR], x2v[NLR], sdx[NLR], sdy[NLR];
     int i0v[NLR];
     using C = Contract<PREC>;
-    typename C::b_t Ereg[NLR][NCG][16];
+    typename C::b_t Ereg[NLR][NA][16];
     int my_ibeg = nsteps, my_imax = 0;
 #pragma unroll
     for (int r = 0; r < NLR; ++r) {
@@ -433,11 +498,11 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
         // B operand for K-group kg: latitude tile_base + 4*kg + lane/16, column lane%16 of group c
         // (E.re, E.im, O.re, O.im = weighted N+S, N+S, (N-S) x, (N-S) x).  Branch-free, every
         // load issued before the first use: the block's first MFMA waits for these.
-        const int col = lane & 15, comp = col & 3;
+        const int col = NF4 ? (lane & 3) : (lane & 15), comp = col & 3;
         const bool keep = nsteps > 0 && !(o.real_only && (comp & 1));
 #pragma unroll
-        for (int c = 0; c < NCG; ++c) {
-            const int f = fbase + 4 * c + (col >> 2);
+        for (int c = 0; c < NA; ++c) {
+            const int f = NF4 ? fbase + c : fbase + 4 * c + (col >> 2);
             const bool fv = keep && f < nfields;
             const double2 *Ff = F + (size_t)(fv ? f : 0) * four_dist + (size_t)im * nlat;
 #pragma unroll
@@ -549,9 +614,10 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             pcz[j] = cf4.z;
             pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
         }
-        typename C::acc_t acc[NCG], acc2[NCG];   // two chains per column group (even / odd K-groups)
+        using accA_t = std::conditional_t<NF4 == 0, typename C::acc_t, double>;
+        accA_t acc[NA], acc2[NA];   // two chains per column group (even / odd K-groups)
 #pragma unroll
-        for (int c = 0; c < NCG; ++c) acc[c] = acc2[c] = C::zero();
+        for (int c = 0; c < NA; ++c) acc[c] = acc2[c] = accA_t{};
         {
             double2 cf[16];
             load_cf16(cf, i);
@@ -568,21 +634,32 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
 #pragma unroll
                 for (int kg = 0; kg < 16; kg += 2)
 #pragma unroll
-                    for (int c = 0; c < NCG; ++c) {
-                        acc[c] = C::mma(a[kg], Ereg[r][c][kg], acc[c]);
-                        acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
+                    for (int c = 0; c < NA; ++c) {
+                        if constexpr (NF4 > 0) {
+                            acc[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[kg], Ereg[r][c][kg], acc[c], 0, 0, 0);
+                            acc2[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[kg + 1], Ereg[r][c][kg + 1], acc2[c], 0, 0, 0);
+                        } else {
+                            acc[c] = C::mma(a[kg], Ereg[r][c][kg], acc[c]);
+                            acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
+                        }
                     }
             }
         }
 #pragma unroll
-        for (int c = 0; c < NCG; ++c) acc[c] += acc2[c];
-        // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
+        for (int c = 0; c < NA; ++c) acc[c] += acc2[c];
         const int pb = b & 1;
         __syncthreads();   // previous block's post-mix has finished reading red
+        if constexpr (NF4 > 0) {
+            // 4x4x4 D: step 4 ((lane >> 2) & 3) + (lane >> 4), column lane & 3 of field c
 #pragma unroll
-        for (int c = 0; c < NCG; ++c)
+            for (int c = 0; c < NA; ++c) red[wave][4 * ((lane >> 2) & 3) + (lane >> 4)][4 * c + (lane & 3)] = acc[c];
+        } else {
+            // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
 #pragma unroll
-            for (int j = 0; j < 4; ++j) red[wave][C::d_row(j, lane)][16 * c + (lane & 15)] = acc[c][j];
+            for (int c = 0; c < NCG; ++c)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) red[wave][C::d_row(j, lane)][16 * c + (lane & 15)] = acc[c][j];
+        }
         __syncthreads();
         // sum over waves + post-mix
 #pragma unroll
@@ -636,19 +713,19 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     }
 }
 
-template <int NW, int NLR, int NCG, int PREC>
+template <int NW, int NLR, int NCG, int PREC, int NF4 = 0>
 void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                  int spec_ltop, hipStream_t s) {
     const DeviceTables *t = a.t;
     dim3 grid((a.mmax + 1) * ((a.nfields + 4 * NCG - 1) / (4 * NCG)));
     // static LDS of the kernel, and the coefficient table if it fits beside it in 160 KB
-    constexpr size_t NCOL = 16 * NCG;
+    constexpr size_t NCOL = NF4 ? 4 * NF4 : 16 * NCG;
     constexpr size_t static_lds = sizeof(double) * (NW * 16 * 65 + NW * 16 * (NCOL + 1) + 2 * NCOL) + sizeof(int) * NW;
     const size_t cf_bytes = sizeof(double2) * (size_t)(a.ltr / 2 + 2);
     const int cf_lds = (static_lds + cf_bytes + 1024 <= 160 * 1024) ? 1 : 0;
     const size_t dyn = cf_lds ? cf_bytes : 0;
-    ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC>, dyn);
-    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC>), grid, dim3(64 * NW), dyn, s, t->x, t->x2, t->st,
+    ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC, NF4>, dyn);
+    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC, NF4>), grid, dim3(64 * NW), dyn, s, t->x, t->x2, t->st,
                        t->wq, t->step_ofs, t->ab, t->pm, a.vector_mode ? t->i0v : t->i0,
                        a.vector_mode ? t->z0v : t->z0, F, four_dist, Q, spec_dist,
                        spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
@@ -676,8 +753,14 @@ void anal_launch(const LegendreArgs &a, const double2 *F, long four_dist, double
     bool wide = (a.nfields > 4) && (nlr <= 3);   // 8 fields per block when registers allow
     if (ncg_want == 1) wide = false;
     if (ncg_want == 2 && nlr <= 3) wide = (a.nfields > 4);
+    // one or two fields (FP64): 4x4x4 blocks, no idle matrix-core columns
+    const int nf4 = (PREC == 0 && a.nfields <= 2 && !synth_force16()) ? a.nfields : 0;
 #define ANAL_CASE(W, R)                                                                              \
     if (nw == W && nlr <= R) {                                                                       \
+        if constexpr (PREC == 0) {                                                                   \
+            if (nf4 == 1) { launch_anal_t<W, R, 1, 0, 1>(a, F, four_dist, Q, spec_dist, spec_ltop, s); return; } \
+            if (nf4 == 2) { launch_anal_t<W, R, 1, 0, 2>(a, F, four_dist, Q, spec_dist, spec_ltop, s); return; } \
+        }                                                                                            \
         if (wide) launch_anal_t<W, R, 2, PREC>(a, F, four_dist, Q, spec_dist, spec_ltop, s);        \
         else launch_anal_t<W, R, 1, PREC>(a, F, four_dist, Q, spec_dist, spec_ltop, s);             \
         return;                                                                                      \
