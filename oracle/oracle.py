@@ -262,3 +262,54 @@ class Oracle:
                 s[idx] = sign * inv * (a @ dP + b @ imP)
                 t[idx] = sign * inv * (a @ imP - b @ dP)
         return s, t
+
+
+# ---- normalizations other than orthonormal (shtns.h:48-55) ----
+NO_CS_PHASE, REAL_NORM = 256 * 4, 256 * 8
+
+
+def norm_factors(lmax, mmax, mres, norm):
+    """c_lm with Y^norm_lm = c_lm Y^orthonormal_lm, in LM order, following legendre_precomp
+    (sht_legendre.c:586-605, 635-660): sht_fourpi (1) multiplies by sqrt(4 pi), sht_schmidt (2)
+    and sht_for_rotations (3) by sqrt(4 pi / (2l+1)); SHT_NO_CS_PHASE drops the (-1)^m and
+    SHT_REAL_NORM scales m > 0 by sqrt(1/2).  Pinned by tests/test_norms.py against the closed
+    forms of these functions built from scipy's associated Legendre functions."""
+    base = norm & 255
+    c = []
+    for im in range(mmax + 1):
+        m = im * mres
+        for l in range(m, lmax + 1):
+            f = 1.0
+            if base == 1:
+                f = np.sqrt(4 * np.pi)
+            elif base in (2, 3):
+                f = np.sqrt(4 * np.pi / (2 * l + 1))
+            if (norm & NO_CS_PHASE) and (m & 1):
+                f = -f
+            if (norm & REAL_NORM) and m > 0:
+                f *= np.sqrt(0.5)
+            c.append(f)
+    return np.array(c)
+
+
+class NormOracle(Oracle):
+    """The oracle in another normalization: synthesis of Q is the orthonormal synthesis of
+    c Q, analysis returns the orthonormal analysis divided by c (the inverse pair)."""
+
+    def __init__(self, lmax, mmax=None, mres=1, nlat=None, nphi=None, grid=0, norm=0):
+        super().__init__(lmax, mmax, mres, nlat, nphi, grid)
+        self.norm = norm
+        self.c = norm_factors(self.lmax, self.mmax, self.mres, norm)
+
+    def synth(self, qlm, ltr=-1):
+        return super().synth(self._spec(qlm) * self.c, ltr)
+
+    def analys(self, vr, ltr=-1):
+        return super().analys(vr, ltr) / self.c
+
+    def synth_sphtor(self, slm, tlm, ltr=-1):
+        return super().synth_sphtor(self._spec(slm) * self.c, self._spec(tlm) * self.c, ltr)
+
+    def analys_sphtor(self, vt, vp, ltr=-1):
+        s, t = super().analys_sphtor(vt, vp, ltr)
+        return s / self.c, t / self.c

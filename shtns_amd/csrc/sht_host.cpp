@@ -257,8 +257,9 @@ extern "C" shtns_cfg shtns_create(int lmax, int mmax, int mres, enum shtns_norm 
     if (lmax > 65534) runerr("lmax too large");
     if (mres <= 0) runerr("MRES must be > 0");
     if (mmax < 0 || mmax * mres > lmax) runerr("MMAX*MRES should not exceed LMAX");
-    if ((int)norm != (int)sht_orthonormal) {
-        set_error(1, "only the orthonormal normalization with Condon-Shortley phase is implemented (norm=%d)", (int)norm);
+    const int nbase = (int)norm & 255;   // sht_orthonormal, sht_fourpi, sht_schmidt, sht_for_rotations
+    if (nbase > (int)sht_for_rotations || ((int)norm & ~(255 | SHT_NO_CS_PHASE | SHT_REAL_NORM))) {
+        set_error(1, "unknown normalization (norm=%d)", (int)norm);
         return nullptr;
     }
     shtns_info *c = new shtns_info();
@@ -277,6 +278,22 @@ extern "C" shtns_cfg shtns_create(int lmax, int mmax, int mres, enum shtns_norm 
             c->mi[lm] = (unsigned short)(im * mres);
             ++lm;
         }
+    // normalization factors relative to orthonormal harmonics with the Condon-Shortley phase
+    // (legendre_precomp, sht_legendre.c:586-605, 635-660: Y_00 = 1 for the 4 pi and Schmidt
+    // norms, Schmidt functions divided by sqrt(2l+1), (-1)^m optional, m > 0 scaled by
+    // sqrt(mpos_renorm) = sqrt(1/2) with SHT_REAL_NORM)
+    if ((int)norm != (int)sht_orthonormal) {
+        c->nrm.resize(c->nlm);
+        for (unsigned i = 0; i < c->nlm; ++i) {
+            const int l = c->li[i], m = c->mi[i];
+            long double f = 1.0L;
+            if (nbase == sht_fourpi) f = sqrtl(4.0L * M_PIl);
+            else if (nbase == sht_schmidt || nbase == sht_for_rotations) f = sqrtl(4.0L * M_PIl / (2 * l + 1));
+            if (((int)norm & SHT_NO_CS_PHASE) && (m & 1)) f = -f;
+            if (((int)norm & SHT_REAL_NORM) && m > 0) f *= sqrtl(0.5L);
+            c->nrm[i] = (double)f;
+        }
+    }
     std::lock_guard<std::mutex> g(g_mutex);
     c->next = g_cfg_list;
     g_cfg_list = c;
@@ -289,12 +306,15 @@ static void free_device(shtns_info *c) {
     if (c->clone_of == nullptr) {
         DeviceTables &d = c->d;
         void *ptrs[] = {d.x, d.x2, d.st, d.wq, d.step_ofs, d.ab, d.pm, d.tm, d.i0, d.z0, d.i0v, d.z0v, d.roots, d.fft_pos,
-                        d.sb_ofs, d.eA, d.abk, d.dblk, d.delta};
+                        d.sb_ofs, d.eA, d.abk, d.dblk, d.delta, d.nrm};
         for (void *p : ptrs)
             if (p) (void)hipFree(p);
     }
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->xscratch) (void)hipFree(c->xscratch);
+    if (c->nscratch) (void)hipFree(c->nscratch);
+    c->nscratch = nullptr;
+    c->nscratch_bytes = 0;
     for (auto &sl : c->hstage)
         if (sl.first) (void)hipFree(sl.first);
     c->hstage.clear();
@@ -593,14 +613,20 @@ extern "C" int shtns_gauss_wts(shtns_cfg c, double *wts) {
     return c->nlat_2;
 }
 
-extern "C" double sh00_1(shtns_cfg) { return sqrt(4. * M_PI); }
-extern "C" double sh10_ct(shtns_cfg) { return sqrt(4. * M_PI / 3.); }
-extern "C" double sh11_st(shtns_cfg) { return -sqrt(2. * M_PI / 3.); }
-extern "C" double shlm_e1(shtns_cfg, int l, int m) {
-    (void)l;
+// normalization factor of (l, m) (1 for orthonormal harmonics with the Condon-Shortley phase)
+static double nrm_of(shtns_cfg c, int l, int m) {
+    if (c->nrm.empty() || m > c->mmax * c->mres || m % c->mres || l < m || l > c->lmax) return 1.0;
+    return c->nrm[LiM(c, l, m / c->mres)];
+}
+// coefficients of 1, cos(theta), sin(theta) cos(phi) and of unit energy (sht_init.c:76-94,
+// 1180-1195): the orthonormal values divided by the normalization factor
+extern "C" double sh00_1(shtns_cfg c) { return sqrt(4. * M_PI) / nrm_of(c, 0, 0); }
+extern "C" double sh10_ct(shtns_cfg c) { return sqrt(4. * M_PI / 3.) / nrm_of(c, 1, 0); }
+extern "C" double sh11_st(shtns_cfg c) { return -sqrt(2. * M_PI / 3.) / nrm_of(c, 1, c->mres == 1 ? 1 : 0); }
+extern "C" double shlm_e1(shtns_cfg c, int l, int m) {
     double x = 1.0;
     if (m != 0) x *= sqrt(0.5);
-    return x;
+    return x / fabs(nrm_of(c, l, m));
 }
 
 // Normalized associated Legendre functions Y_l^m(x), l = m..lmax (GSL-compatible output,
@@ -611,7 +637,10 @@ extern "C" int legendre_sphPlm_array(shtns_cfg c, const int lmax, const int im, 
     const double s2 = (1. - x) * (1. + x);
     long double y0 = seed_amm(m) * powl(sqrtl((long double)s2), m);
     yl[0] = (double)y0;
-    if (lmax == m) return lmax - m + 1;
+    if (lmax == m) {
+        yl[0] *= nrm_of(c, m, m);
+        return 1;
+    }
     long double y1 = sqrtl(2.0L * m + 3) * x * y0;
     yl[1] = (double)y1;
     for (int l = m + 2; l <= lmax; ++l) {
@@ -624,6 +653,8 @@ extern "C" int legendre_sphPlm_array(shtns_cfg c, const int lmax, const int im, 
         y1 = y2;
         yl[l - m] = (double)y1;
     }
+    if (!c->nrm.empty())
+        for (int l = m; l <= lmax; ++l) yl[l - m] *= nrm_of(c, l, m);
     return lmax - m + 1;
 }
 
@@ -701,8 +732,8 @@ extern "C" int legendre_sphPlm_deriv_array(shtns_cfg c, const int lmax, const in
     plm_deriv(m, lmax, x, ys, dy);
     int lnz = lmax + 1;
     for (int l = m; l <= lmax; ++l) {
-        yl[l - m] = (double)ys[l - m];
-        dyl[l - m] = (double)dy[l - m];
+        yl[l - m] = (double)ys[l - m] * nrm_of(c, l, m);
+        dyl[l - m] = (double)dy[l - m] * nrm_of(c, l, m);
         if (lnz > lmax && (yl[l - m] != 0.0 || dyl[l - m] != 0.0)) lnz = l;
     }
     return lnz;
@@ -721,8 +752,9 @@ extern "C" void mul_ct_matrix(shtns_cfg c, double *mx) {
             const double hi = (l < c->lmax)
                                   ? sqrt((double)((long)(l + 1) * (l + 1) - m * m) / ((2.0 * l + 1) * (2.0 * l + 3)))
                                   : 0.0;
-            mx[2 * lm] = lo;
-            mx[2 * lm + 1] = hi;
+            // in the configuration's normalization: nrm_{l-1} / nrm_l and nrm_{l+1} / nrm_l
+            mx[2 * lm] = (l > m) ? lo * nrm_of(c, l - 1, (int)m) / nrm_of(c, l, (int)m) : 0.0;
+            mx[2 * lm + 1] = (l < c->lmax) ? hi * nrm_of(c, l + 1, (int)m) / nrm_of(c, l, (int)m) : 0.0;
         }
     }
 }
@@ -815,10 +847,22 @@ static inline double re_e(const double *F, int m, double cm, double sm) {
     return (m == 0 ? 1.0 : 2.0) * (F[0] * cm - F[1] * sm);
 }
 
+// host spectrum times the normalization factors (Q itself for orthonormal harmonics)
+static const cplx *host_scaled(shtns_cfg c, const cplx *Q, std::vector<cplx> &buf) {
+    if (c->nrm.empty() || Q == nullptr) return Q;
+    buf.assign(Q, Q + c->nlm);
+    for (unsigned i = 0; i < c->nlm; ++i) buf[i] *= c->nrm[i];
+    return buf.data();
+}
+
 // vr = sum Q Y;  vt = dS/dtheta + (1/sin) dT/dphi;  vp = (1/sin) dS/dphi - dT/dtheta
 // (sht_func.c:672-715); SH_to_grad_point is the T = 0 case with Q the radial derivative
 static void qst_point(shtns_cfg c, const cplx *Q, const cplx *S, const cplx *T, double cost, double phi, double *vr,
                       double *vt, double *vp) {
+    std::vector<cplx> bq, bs, bt;
+    Q = host_scaled(c, Q, bq);
+    S = host_scaled(c, S, bs);
+    T = host_scaled(c, T, bt);
     const double s = sqrt((1. - cost) * (1. + cost));
     std::vector<long double> ys, dy;
     double r = 0, t = 0, p = 0;
@@ -851,6 +895,10 @@ extern "C" void SH_to_grad_point(shtns_cfg c, cplx *DrSlm, cplx *Slm, double cos
 // directly, with each angle reduced exactly (m j mod nphi).
 static void qst_lat(shtns_cfg c, const cplx *Q, const cplx *S, const cplx *T, double cost, double *vr, double *vt,
                     double *vp, int nphi, int ltr, int mtr) {
+    std::vector<cplx> bq, bs, bt;
+    Q = host_scaled(c, Q, bq);
+    S = host_scaled(c, S, bs);
+    T = host_scaled(c, T, bt);
     if (nphi <= 0) return;
     if (ltr > c->lmax) ltr = c->lmax;
     if (mtr > c->mmax) mtr = c->mmax;
@@ -958,7 +1006,7 @@ static int device_init(shtns_info *c) {
         upload(&d.wq, wq.data(), nl2) || upload(&d.step_ofs, ofs.data(), ofs.size()) ||
         upload(&d.ab, ab.data(), ab.size()) || upload(&d.pm, pm.data(), pm.size()) ||
         upload(&d.roots, roots.data(), roots.size()) || upload(&d.fft_pos, pos.data(), pos.size()) ||
-        upload(&d.tm, c->tm.data(), c->tm.size())) {
+        upload(&d.tm, c->tm.data(), c->tm.size()) || (!c->nrm.empty() && upload(&d.nrm, c->nrm.data(), c->nrm.size()))) {
         set_error(2, "device table allocation failed");
         c->gpu_ready = true;
         free_device(c);
@@ -1023,6 +1071,8 @@ extern "C" shtns_cfg cushtns_clone(shtns_cfg c, hipStream_t compute_stream, hipS
     k->scratch = nullptr;
     k->xscratch = nullptr;
     k->xscratch_bytes = 0;
+    k->nscratch = nullptr;
+    k->nscratch_bytes = 0;
     k->scratch_bytes = 0;
     k->own_streams = false;
     k->comp_stream = compute_stream;
@@ -1197,6 +1247,43 @@ static void *xscratch(shtns_cfg c, size_t bytes) {
     return c->xscratch;
 }
 
+static void *nscratch(shtns_cfg c, size_t bytes) {
+    if (bytes <= c->nscratch_bytes) return c->nscratch;
+    if (c->nscratch) {
+        (void)hipStreamSynchronize(c->comp_stream);
+        (void)hipFree(c->nscratch);
+    }
+    c->nscratch = nullptr;
+    c->nscratch_bytes = 0;
+    if (hipMalloc(&c->nscratch, bytes) != hipSuccess) {
+        set_error(2, "normalization staging of %zu bytes failed", bytes);
+        return nullptr;
+    }
+    c->nscratch_bytes = bytes;
+    return c->nscratch;
+}
+
+// Non-orthonormal normalizations act on the spectra only: the transforms run on orthonormal
+// harmonics with Q nrm (synthesis; Q / nrm for the adjoint of the analysis) as input, and the
+// analysis output is divided by nrm (multiplied for the adjoint of the synthesis).
+// scaled_input: a scaled copy of nf spectra (slot 0 or 1 of the staging), or Q itself
+static const cplx *scaled_input(shtns_cfg c, const cplx *Q, long &spec_dist, int nf, int inv, int slot) {
+    if (c->nrm.empty() || nf <= 0) return Q;
+    const size_t one = (size_t)nf * c->nlm * sizeof(double2);
+    char *b = (char *)nscratch(c, 2 * one);
+    if (!b) return nullptr;
+    double2 *out = reinterpret_cast<double2 *>(b + slot * one);
+    launch_spec_scale(reinterpret_cast<const double2 *>(Q), spec_dist, out, c->nlm, c->nlm, nf, c->d.nrm, inv,
+                      c->comp_stream);
+    spec_dist = c->nlm;
+    return reinterpret_cast<const cplx *>(out);
+}
+static void scale_output(shtns_cfg c, cplx *Q, long spec_dist, int nf, int inv) {
+    if (c->nrm.empty() || nf <= 0 || g_last_err) return;
+    launch_spec_scale(reinterpret_cast<const double2 *>(Q), spec_dist, reinterpret_cast<double2 *>(Q), spec_dist,
+                      c->nlm, nf, c->d.nrm, inv, c->comp_stream);
+}
+
 // synthesis core: Q (nfields, spec_dist) -> V (nfields, spat_dist)
 // The Legendre kernel runs in chunks of latitude tiles on the compute stream; the FFT of a
 // chunk's latitudes (north and south rows) runs on the second stream as soon as the chunk
@@ -1298,10 +1385,24 @@ static size_t synth_scratch_bytes(shtns_cfg c, int nfields) {
 // ext != nullptr: a caller-provided workspace of at least synth_scratch_bytes(c, nfields)
 // bytes is used instead of the configuration's scratch; no stream, event or memory is then
 // created (no profiling, no overlapped chunks).
+static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
+                          double mpos_scale, int lat_weight, int prec, float *vf, double *vstage, void *ext);
 static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
                        double mpos_scale, int lat_weight, int prec, float *vf = nullptr, double *vstage = nullptr,
                        void *ext = nullptr) {
     if (nfields <= 0) return;
+    if (!c->nrm.empty()) {
+        if (ext) {
+            set_error(1, "a non-orthonormal normalization needs a scaled copy of the spectra (allocating entry points only)");
+            return;
+        }
+        Q = scaled_input(c, Q, spec_dist, nfields, lat_weight /* adjoint of the analysis: 1 / nrm */, 0);
+        if (!Q) return;
+    }
+    synth_core_on(c, Q, V, ltr, nfields, spec_dist, spat_dist, mpos_scale, lat_weight, prec, vf, vstage, ext);
+}
+static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
+                          double mpos_scale, int lat_weight, int prec, float *vf, double *vstage, void *ext) {
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
@@ -1395,10 +1496,17 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
 // stream ahead of the Legendre analysis of the previous chunk on the compute stream.
 // V is double (vf == nullptr) or float (vf != nullptr, vstage as in synth_core: when set,
 // vf is first converted into it).
+static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
+                         double mpos_scale, int lat_weight, int prec, const float *vf, double *vstage);
 static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
                       double mpos_scale, int lat_weight, int prec, const float *vf = nullptr,
                       double *vstage = nullptr) {
     if (nfields <= 0) return;
+    anal_core_on(c, V, Q, ltr, nfields, spat_dist, spec_dist, mpos_scale, lat_weight, prec, vf, vstage);
+    scale_output(c, Q, spec_dist, nfields, lat_weight /* analysis: 1 / nrm; adjoint of the synthesis: nrm */);
+}
+static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
+                         double mpos_scale, int lat_weight, int prec, const float *vf, double *vstage) {
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
@@ -1461,6 +1569,12 @@ static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfield
 static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, double *Vp, int ltr, int nfields,
                         long spec_dist, long spat_dist) {
     if (nfields <= 0) return;
+    if (!c->nrm.empty()) {
+        long d2 = spec_dist;
+        S = scaled_input(c, S, spec_dist, nfields, 0, 0);
+        T = scaled_input(c, T, d2, nfields, 0, 1);
+        if (!S || !T) return;
+    }
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
@@ -1515,6 +1629,8 @@ static void vanal_core(shtns_cfg c, const double *Vt, const double *Vp, cplx *S,
     launch_vw_to_sphtor(VW, vw_dist, reinterpret_cast<double2 *>(S), reinterpret_cast<double2 *>(T), spec_dist,
                         c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);
     HIP_OK(hipGetLastError());
+    scale_output(c, S, spec_dist, nfields, 1);
+    scale_output(c, T, spec_dist, nfields, 1);
 }
 
 // ---- device-pointer API ----
@@ -2015,11 +2131,28 @@ static bool ml_prepare(shtns_cfg c, int im, int ltr, const char *who, MlStage &g
     }
     return true;
 }
+// one-order spectra (degrees m + j, j < n) times / divided by the normalization factors
+static cplx *ml_scaled(shtns_cfg c, int im, cplx *Ql, int n, std::vector<cplx> &buf) {
+    if (c->nrm.empty() || n <= 0) return Ql;
+    const long b = LiM(c, 0, im);
+    const int m = im * c->mres;
+    buf.assign(Ql, Ql + n);
+    for (int j = 0; j < n; ++j) buf[j] *= c->nrm[b + m + j];
+    return buf.data();
+}
+static void ml_unscale(shtns_cfg c, int im, cplx *Ql, int n) {
+    if (c->nrm.empty() || g_last_err) return;
+    const long b = LiM(c, 0, im);
+    const int m = im * c->mres;
+    for (int j = 0; j < n; ++j) Ql[j] /= c->nrm[b + m + j];
+}
 extern "C" void SH_to_spat_ml(shtns_cfg c, int im, cplx *Ql, cplx *Vr, int ltr) {
     clear_error();
     MlStage g{HostStage{c, {}}};
     if (!ml_prepare(c, im, ltr, "SH_to_spat_ml", g)) return;
     const hipStream_t s = c->comp_stream;
+    std::vector<cplx> qb;
+    Ql = ml_scaled(c, im, Ql, g.nl, qb);
     if (g.nl > 0) {
         if (!h2d(g.Q, Ql, (size_t)g.nl * sizeof(cplx), s)) { set_error(2, "SH_to_spat_ml: copy failed"); return; }
         launch_ml_synth(g.T, c->nlat, g.nl, g.Q, g.m == 0, g.V, s);
@@ -2043,6 +2176,7 @@ extern "C" void spat_to_SH_ml(shtns_cfg c, int im, cplx *Vr, cplx *Ql, int ltr) 
     HIP_OK(hipGetLastError());
     if (!d2h(Ql, g.Q, (size_t)g.nq * sizeof(cplx), s)) { set_error(2, "spat_to_SH_ml: copy failed"); return; }
     HIP_OK(hipStreamSynchronize(s));
+    ml_unscale(c, im, Ql, g.nq);
 }
 
 /* Vector Legendre transforms at one order m = im*mres (shtns.h:336-337):
@@ -2056,6 +2190,9 @@ extern "C" void SHsphtor_to_spat_ml(shtns_cfg c, int im, cplx *Sl, cplx *Tl, cpl
     if (!ml_prepare(c, im, ltr, "SHsphtor_to_spat_ml", g, true)) return;
     const hipStream_t s = c->comp_stream;
     const size_t vb = (size_t)c->nlat * sizeof(double2);
+    std::vector<cplx> sb, tb;
+    Sl = ml_scaled(c, im, Sl, g.nl, sb);
+    Tl = ml_scaled(c, im, Tl, g.nl, tb);
     if (g.nl > 0) {
         if (!h2d(g.Q, Sl, (size_t)g.nl * sizeof(cplx), s) || !h2d(g.Q2, Tl, (size_t)g.nl * sizeof(cplx), s)) {
             set_error(2, "SHsphtor_to_spat_ml: copy failed");
@@ -2085,6 +2222,8 @@ extern "C" void spat_to_SHsphtor_ml(shtns_cfg c, int im, cplx *Vt, cplx *Vp, cpl
     HIP_OK(hipGetLastError());
     if (!d2h(Sl, g.Q, qb, s) || !d2h(Tl, g.Q2, qb, s)) { set_error(2, "spat_to_SHsphtor_ml: copy failed"); return; }
     HIP_OK(hipStreamSynchronize(s));
+    ml_unscale(c, im, Sl, g.nq);
+    ml_unscale(c, im, Tl, g.nq);
 }
 
 // shorthands of the one-order transforms (shtns.h:338-342): a zero spectrum for the missing

@@ -62,6 +62,9 @@ struct DeviceTables {
     // (D(b, 0), D(b, 1)), delta[step_ofs[slot] + i] = D(block of i, i & 1)
     double2 *abk = nullptr;
     int2 *dblk = nullptr;
+    // per-(l, m) factor c_lm of a non-orthonormal normalization (null: orthonormal with the
+    // Condon-Shortley phase), index lm as the spectra
+    double *nrm = nullptr;
     int *delta = nullptr;
     // FFT: roots e^{2 pi i k/nphi}, k < nphi
     double2 *roots = nullptr;
@@ -116,6 +119,13 @@ struct shtns_info {
     size_t xscratch_bytes = 0;
     // kernel shapes picked by cushtns_tune (0: automatic)
     int tune_synth_ncg = 0, tune_anal_nw = 0, tune_anal_ncg = 0, tune_fft_synth = 0, tune_fft_anal = 0;
+    // normalization (shtns_create's norm argument): Y^norm_lm = nrm[lm] Y^orthonormal_lm with
+    // nrm = sqrt(4 pi) (sht_fourpi) or sqrt(4 pi / (2l+1)) (sht_schmidt), times (-1)^m without
+    // the Condon-Shortley phase and sqrt(1/2) for m > 0 with SHT_REAL_NORM; empty for the
+    // orthonormal default.  Synthesis uses Q nrm, analysis returns Q / nrm.
+    std::vector<double> nrm;
+    void *nscratch = nullptr;   // scaled copies of input spectra (grown on demand)
+    size_t nscratch_bytes = 0;
     // SHT_LOAD_SAVE_CFG given to shtns_set_grid: kernel shapes are read from / appended to
     // the tuning file; grid_flags = the other flags of that call (part of the file's key)
     bool load_save = false;
@@ -123,6 +133,10 @@ struct shtns_info {
 };
 
 namespace shtns_amd {
+
+// out[f][lm] = in[f][lm] * nrm[lm] (inv = 0) or / nrm[lm] (inv = 1), f < nf (sht_kernels.hip)
+void launch_spec_scale(const double2 *in, long in_dist, double2 *out, long out_dist, int nlm, int nf,
+                       const double *nrm, int inv, hipStream_t s);
 
 // tuning file (sht_batch.cpp): apply the entry saved for this configuration and batch size
 // (nfields <= 0: the last entry for this configuration, any batch size); true if one was found

@@ -168,6 +168,23 @@ __global__ __launch_bounds__(256) void premix_kernel(const int *__restrict__ ste
     }
 }
 
+__global__ void spec_scale_kernel(const double2 *__restrict__ in, long in_dist, double2 *__restrict__ out,
+                                  long out_dist, int nlm, const double *__restrict__ nrm, int inv) {
+    const int lm = blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (lm >= nlm) return;
+    const double c = inv ? 1.0 / nrm[lm] : nrm[lm];
+    const double2 v = in[(size_t)f * in_dist + lm];
+    out[(size_t)f * out_dist + lm] = make_double2(v.x * c, v.y * c);
+}
+
+void launch_spec_scale(const double2 *in, long in_dist, double2 *out, long out_dist, int nlm, int nf,
+                       const double *nrm, int inv, hipStream_t s) {
+    if (nf <= 0 || nlm <= 0) return;
+    hipLaunchKernelGGL(spec_scale_kernel, dim3((nlm + 255) / 256, nf), dim3(256), 0, s, in, in_dist, out, out_dist, nlm,
+                       nrm, inv);
+}
+
 bool anal_shape_ok(int nw, int nlr) {
     // the (waves, tiles per wave) cases compiled in anal_launch (sht_legendre.h)
     static const int maxr[13] = {0, 6, 6, 6, 6, 6, 6, 6, 6, 1, 1, 1, 1};

@@ -335,7 +335,8 @@ static void cfg_rotate(shtns_cfg c, cplx *Q, cplx *R, double a, double b, double
         fprintf(stderr, "*** [shtns_amd] %s: truncature makes rotation not closed.\n", who);
         exit(1);
     }
-    shtns_rot r = shtns_rotation_create(c->lmax, c->mmax, 0);
+    // factors depending on l only commute with rotations: only the phase and real-norm flags matter
+    shtns_rot r = shtns_rotation_create(c->lmax, c->mmax, c->norm & (SHT_NO_CS_PHASE | SHT_REAL_NORM));
     shtns_rotation_set_angles_ZYZ(r, a, b, g);
     shtns_rotation_apply_real(r, Q, R);
     shtns_rotation_destroy(r);
