@@ -45,8 +45,10 @@ def fft_flops(nlat, nphi):
 
 
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sht_pair_l1023_b16_hbm_traffic.txt")
-TRAFFIC_ROWS = {"legendre_synth": ("premix_kernel", "legendre_synth_kernel"),
-                "legendre_anal": ("legendre_anal_kernel",), "fft_synth": ("fft_synth",), "fft_anal": ("fft_anal",)}
+# rows of the FP64 kernels of this workload in that summary
+TRAFFIC_ROWS = {"legendre_synth": ("premix_kernel<64>", "legendre_synth_kernel<4, 0, 0>"),
+                "legendre_anal": ("legendre_anal_kernel<12, 1, 2, 0, 0>",), "fft_synth": ("fft_synth",),
+                "fft_anal": ("fft_anal",)}
 
 
 def profiled_traffic(kernel):
