@@ -1510,8 +1510,15 @@ static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfi
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
-    double2 *F = (double2 *)scratch(c, (size_t)nfields * four_dist * sizeof(double2));
-    if (!F) return;
+    LegendreArgs a0 = leg_args(c, ltr, imlim, nfields);
+    a0.prec = prec;
+    const int nsplit = anal_splits(a0);   // latitude-split analysis: partial spectra after F
+    const size_t fbytes = (size_t)nfields * four_dist * sizeof(double2);
+    const size_t pbytes = (nsplit > 1) ? (size_t)nsplit * nfields * c->nlm * sizeof(double2) : 0;
+    char *fb = (char *)scratch(c, fbytes + pbytes);
+    if (!fb) return;
+    double2 *F = (double2 *)fb;
+    double2 *part = pbytes ? (double2 *)(fb + fbytes) : nullptr;
     long sp, st;
     spat_strides(c, sp, st);
     const hipStream_t S = c->comp_stream;
@@ -1537,6 +1544,8 @@ static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfi
         a.mpos_scale = mpos_scale;
         a.lat_weight = lat_weight;
         a.prec = prec;
+        a.part = part;
+        a.part_dist = c->nlm;
         ProfScope ps(c, PK_LEG_ANAL);
         launch_legendre_anal(a, F, four_dist, reinterpret_cast<double2 *>(Q), spec_dist, c->lmax, S);
         HIP_OK(hipGetLastError());
@@ -1611,20 +1620,24 @@ static void vanal_core(shtns_cfg c, const double *Vt, const double *Vp, cplx *S,
     const long vw_dist = nlm_calc(c->lmax + 1, c->mmax, c->mres) + 2;
     const size_t fbytes = (size_t)2 * nfields * four_dist * sizeof(double2);
     const size_t vbytes = (size_t)2 * nfields * vw_dist * sizeof(double2);
-    char *base = (char *)scratch(c, fbytes + vbytes);
+    LegendreArgs a = leg_args(c, ltr + 1, imlim, 2 * nfields);
+    a.vector_mode = 1;
+    a.lat_weight = 1;
+    a.prec = prec_f64(c);
+    const int nsplit = anal_splits(a);
+    const size_t pbytes = (nsplit > 1) ? (size_t)nsplit * 2 * nfields * vw_dist * sizeof(double2) : 0;
+    char *base = (char *)scratch(c, fbytes + vbytes + pbytes);
     if (!base) return;
     double2 *F = (double2 *)base;
     double2 *VW = (double2 *)(base + fbytes);
+    a.part = pbytes ? (double2 *)(base + fbytes + vbytes) : nullptr;
+    a.part_dist = vw_dist;
     long sp, st;
     spat_strides(c, sp, st);
     launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, Vt, spat_dist, F, 2 * four_dist, sp, st,
                     c->comp_stream);
     launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, Vp, spat_dist, F + four_dist, 2 * four_dist, sp,
                     st, c->comp_stream);
-    LegendreArgs a = leg_args(c, ltr + 1, imlim, 2 * nfields);
-    a.vector_mode = 1;
-    a.lat_weight = 1;
-    a.prec = prec_f64(c);
     launch_legendre_anal(a, F, four_dist, VW, vw_dist, c->lmax + 1, c->comp_stream);
     launch_vw_to_sphtor(VW, vw_dist, reinterpret_cast<double2 *>(S), reinterpret_cast<double2 *>(T), spec_dist,
                         c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);

@@ -155,7 +155,16 @@ struct LegendreArgs {
     int synth_ncg = 0;        // synthesis field groups of 4 per wave: 1, 2 or 4
     int anal_nw = 0;          // analysis waves per block (divides the number of latitude tiles)
     int anal_ncg = 0;         // analysis field groups of 4 per block: 1 or 2
+    // latitude-split analysis (anal_splits > 1): per-split partial spectra, part_dist doubles2
+    // per field, summed into the output by a second kernel
+    double2 *part = nullptr;
+    long part_dist = 0;
 };
+
+// number of latitude splits of the analysis for these arguments (1: none).  With few fields
+// and many latitude tiles an order's tiles are split over several blocks of 12 waves (one
+// tile each); the caller then provides part (splits x nfields x part_dist double2).
+int anal_splits(const LegendreArgs &a);
 
 // status of the last C-API call of this thread (shtns_last_error); set_error also prints
 void set_error(int code, const char *fmt, ...);

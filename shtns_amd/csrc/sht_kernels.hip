@@ -185,6 +185,33 @@ void launch_spec_scale(const double2 *in, long in_dist, double2 *out, long out_d
                        nrm, inv);
 }
 
+int anal_splits(const LegendreArgs &a) {
+    static const bool off = getenv("SHTNS_AMD_ANAL_SPLIT") && atoi(getenv("SHTNS_AMD_ANAL_SPLIT")) == 0;
+    const int ntiles = (a.nlat_2 + 63) / 64;
+    if (off || synth_force16() || a.prec != kPrecF64 || a.nfields > 2 || ntiles <= 12 || ntiles % 12 != 0) return 1;
+    return ntiles / 12;
+}
+
+__global__ void sum_splits_kernel(const double2 *__restrict__ part, long part_dist, int nsplit, int nfields,
+                                  double2 *__restrict__ Q, long spec_dist, long n) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int f = blockIdx.y;
+    if (i >= n) return;
+    double2 acc = part[(size_t)f * part_dist + i];
+    for (int k = 1; k < nsplit; ++k) {   // fixed order: deterministic
+        const double2 v = part[((size_t)k * nfields + f) * part_dist + i];
+        acc.x += v.x;
+        acc.y += v.y;
+    }
+    Q[(size_t)f * spec_dist + i] = acc;
+}
+
+void launch_sum_splits(const double2 *part, long part_dist, int nsplit, int nfields, double2 *Q, long spec_dist,
+                       long n, hipStream_t s) {
+    hipLaunchKernelGGL(sum_splits_kernel, dim3((unsigned)((n + 255) / 256), nfields), dim3(256), 0, s, part,
+                       part_dist, nsplit, nfields, Q, spec_dist, n);
+}
+
 bool anal_shape_ok(int nw, int nlr) {
     // the (waves, tiles per wave) cases compiled in anal_launch (sht_legendre.h)
     static const int maxr[13] = {0, 6, 6, 6, 6, 6, 6, 6, 6, 1, 1, 1, 1};

@@ -429,7 +429,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const double4 *__restrict__ pm, const int *__restrict__ i0tab, const double2 *__restrict__ z0tab,
     const double2 *__restrict__ F, long four_dist, double2 *__restrict__ Q, long spec_dist, int lmax_spec,
     int spec_ltop, int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
-    double mpos_scale, int lat_weight, int cf_lds) {
+    double mpos_scale, int lat_weight, int cf_lds, int tile_base) {
     static_assert(NF4 == 0 || (PREC == 0 && NCG == 1 && NF4 <= 3), "4x4x4 path: FP64, one group of <= 3 fields");
     constexpr int NA = NF4 ? NF4 : NCG;           // operand / accumulator sets
     constexpr int NCOL = NF4 ? 4 * NF4 : 16 * NCG;
@@ -478,7 +478,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     int my_ibeg = nsteps, my_imax = 0;
 #pragma unroll
     for (int r = 0; r < NLR; ++r) {
-        const int k = (wave + NW * r) * 64 + lane;
+        const int k = (tile_base + wave + NW * r) * 64 + lane;
         const bool kv = (k < nlat_2);
         const int kk = kv ? k : 0;
         x2v[r] = x2k[kk];
@@ -507,7 +507,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             const double2 *Ff = F + (size_t)(fv ? f : 0) * four_dist + (size_t)im * nlat;
 #pragma unroll
             for (int kg = 0; kg < 16; ++kg) {
-                const int kl = (wave + NW * r) * 64 + 4 * kg + (lane >> 4);
+                const int kl = (tile_base + wave + NW * r) * 64 + 4 * kg + (lane >> 4);
                 const bool kv = fv && kl < nlat_2;
                 if constexpr (NLR >= 3) {
                     // several tiles per wave: one K-group at a time (loads hoisted together
@@ -715,7 +715,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
 
 template <int NW, int NLR, int NCG, int PREC, int NF4 = 0>
 void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
-                 int spec_ltop, hipStream_t s) {
+                 int spec_ltop, hipStream_t s, int tile_base = 0) {
     const DeviceTables *t = a.t;
     dim3 grid((a.mmax + 1) * ((a.nfields + 4 * NCG - 1) / (4 * NCG)));
     // static LDS of the kernel, and the coefficient table if it fits beside it in 160 KB
@@ -729,13 +729,29 @@ void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, doub
                        t->wq, t->step_ofs, t->ab, t->pm, a.vector_mode ? t->i0v : t->i0,
                        a.vector_mode ? t->z0v : t->z0, F, four_dist, Q, spec_dist,
                        spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
-                       a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale, a.lat_weight, cf_lds);
+                       a.ltr, a.imlim, a.nfields, a.vector_mode, a.mpos_scale, a.lat_weight, cf_lds, tile_base);
 }
+
+// Q[f][i] = sum_s part[s][f][i], i < n (the latitude splits' partial spectra)
+void launch_sum_splits(const double2 *part, long part_dist, int nsplit, int nfields, double2 *Q, long spec_dist,
+                       long n, hipStream_t s);
 
 template <int PREC>
 void anal_launch(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                  int spec_ltop, hipStream_t s) {
     const int ntiles = (a.nlat_2 + 63) / 64;
+    if constexpr (PREC == 0) {
+        const int ns = anal_splits(a);
+        if (ns > 1 && a.part) {   // 12-wave blocks over a latitude split each, then the sum
+            for (int k = 0; k < ns; ++k) {
+                double2 *P = a.part + (size_t)k * a.nfields * a.part_dist;
+                if (a.nfields == 1) launch_anal_t<12, 1, 1, 0, 1>(a, F, four_dist, P, a.part_dist, spec_ltop, s, 12 * k);
+                else launch_anal_t<12, 1, 1, 0, 2>(a, F, four_dist, P, a.part_dist, spec_ltop, s, 12 * k);
+            }
+            launch_sum_splits(a.part, a.part_dist, ns, a.nfields, Q, spec_dist, a.part_dist, s);
+            return;
+        }
+    }
     // waves per block: one latitude tile per wave up to 12 waves (measured fastest: lowest
     // register count, 3 waves per SIMD); beyond, a divisor of the tile count, at most 8
     int nw = 1;
