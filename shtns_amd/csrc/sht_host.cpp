@@ -982,7 +982,7 @@ static int device_init(shtns_info *c) {
     for (int slot = 0; slot <= mmax + 1; ++slot) {
         const int m = (slot <= mmax) ? slot * mres : 1;
         ofs[slot] = tot;
-        tot += (L + 1 - m) / 2 + 2;
+        tot += (L + 1 - m) / 2 + 8;   // rows continue past the last degree: block loads need no clamp
     }
     ofs[mmax + 2] = tot;
     std::vector<double2> ab(tot);

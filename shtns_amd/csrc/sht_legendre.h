@@ -228,10 +228,14 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
         if (i0 == ibeg) { za = sd.x; zb = sd.y; }
 
         // advance 4 recurrence steps starting at step i, writing z_i..z_{i+3} to tile buffer
-        auto advance = [&](int i, int buf) {
-            double2 cf[4];
+        // coefficients of steps i .. i+3: the table holds at least nsteps + 7 rows per order
+        // (continued past the last degree), so no clamp is needed and the four pairs are one
+        // 64-byte scalar load
+        auto load_cf = [&](int i, double2 *cf) {
 #pragma unroll
-            for (int s = 0; s < 4; ++s) cf[s] = abm[min(i + s, nsteps - 1)];
+            for (int s = 0; s < 4; ++s) cf[s] = abm[i + s];
+        };
+        auto advance = [&](int i, int buf, const double2 *cf) {
             if (i <= wmax) {   // some lane of this wave starts in this block
 #pragma unroll
                 for (int s = 0; s < 4; ++s) {
@@ -251,49 +255,65 @@ __global__ __launch_bounds__(256) void legendre_synth_kernel(
                 }
             }
         };
-        auto load_b = [&](int i, double *b) {
+        // B operand of column group (or field) c for the block at step i
+        // (a 32-bit byte offset from the wave-uniform base: one address register)
+        auto load_b1 = [&](int i, int c) {
             const int ii = min(i + (lane >> 4), slot_rows - 1);   // rows >= nsteps hold zeros
-            if constexpr (NF4 > 0) {
-#pragma unroll
-                for (int c = 0; c < NA; ++c) b[c] = qb[(size_t)ii * NCOL + c * 4 + (lane & 3)];
-            } else {
-#pragma unroll
-                for (int c = 0; c < NCG; ++c) b[c] = qb[(size_t)ii * NCOL + c * 16 + (lane & 15)];
-            }
+            const unsigned e = NF4 > 0 ? (unsigned)(ii * NCOL + c * 4 + (lane & 3)) : (unsigned)(ii * NCOL + c * 16 + (lane & 15));
+            return *(const double *)((const char *)qb + 8u * e);
         };
 
-        double bcur[NA], bnxt[NA];
-        advance(ibeg, 0);
-        load_b(ibeg, bcur);
+        double bcur[NA];
+        {
+            double2 cf0[4];
+            load_cf(ibeg, cf0);
+            advance(ibeg, 0, cf0);
+        }
+        // in column-group order, as in the loop (the first MFMAs then wait for the first load only)
+#pragma unroll
+        for (int c = 0; c < NA; ++c) {
+            bcur[c] = load_b1(ibeg, c);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+        // MFMAs of column group (or field) c on this block's A operands
+        auto mma_group = [&](int c, const typename C::a_t *a) {
+            if constexpr (NF4 > 0) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[r][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[r], bcur[c], acc[r][c], 0, 0, 0);
+            } else {
+                const typename C::b_t bb = C::split(bcur[c]);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    if constexpr (PREC == 0) acc[r][c] = C::mma(a[r], bb, acc[r][c]);
+                    else part[r][c] = C::mma(a[r], bb, part[r][c]);
+                }
+            }
+        };
         int buf = 0, blk = 0;
         for (int i = ibeg; i < nsteps; i += 4, ++blk) {
-            load_b(i + 4, bnxt);
             __builtin_amdgcn_s_waitcnt(0xc07f);   // LDS tile of this block written
             __builtin_amdgcn_wave_barrier();
             typename C::a_t a[4];
 #pragma unroll
             for (int r = 0; r < 4; ++r) a[r] = C::load_z(&ytile[wave][buf][lane >> 4][r * 16 + (lane & 15)]);
-            if constexpr (NF4 > 0) {
+            mma_group(0, a);
+            // The next block's coefficients: a scalar load issued behind the first MFMAs (scalar
+            // and LDS loads share one counter, so issued before the A reads it would delay
+            // them), in flight while the other MFMAs issue.
+            __builtin_amdgcn_sched_barrier(0);
+            double2 cfn[4];
+            load_cf(i + 4, cfn);
+            bcur[0] = load_b1(i + 4, 0);   // next block's B operands, each once its MFMAs have issued
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-                for (int c = 0; c < NA; ++c)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r)
-                        acc[r][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[r], bcur[c], acc[r][c], 0, 0, 0);
-            } else {
-#pragma unroll
-                for (int c = 0; c < NCG; ++c) {
-                    const typename C::b_t bb = C::split(bcur[c]);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        if constexpr (PREC == 0) acc[r][c] = C::mma(a[r], bb, acc[r][c]);
-                        else part[r][c] = C::mma(a[r], bb, part[r][c]);
-                    }
-                }
+            for (int c = 1; c < NA; ++c) {
+                mma_group(c, a);
+                bcur[c] = load_b1(i + 4, c);
+                __builtin_amdgcn_sched_barrier(0);
             }
-            if (i + 4 < nsteps) advance(i + 4, buf ^ 1);
+            advance(i + 4, buf ^ 1, cfn);   // also after the last block (in bounds, unused)
             buf ^= 1;
-#pragma unroll
-            for (int c = 0; c < NA; ++c) bcur[c] = bnxt[c];
             if ((blk & 3) == 3) flush();
         }
         flush();
