@@ -442,8 +442,11 @@ void synth_tiles_launch(const LegendreArgs &a, int spec_ltop, double2 *F, long f
 // (four 4x4x4 blocks = 16 steps x 4 latitudes x the field's 4 columns; the A operand is the
 // 16x16x4 one, B lane l holds column l & 3 of latitude 4 kg + (l >> 4) in every block, and
 // D lane l holds step 4 ((l >> 2) & 3) + (l >> 4), column l & 3).
-template <int NW, int NLR, int NCG, int PREC, int NF4 = 0>
-__global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
+// NG > 1 (one tile per wave): the block holds NG field groups of the same order, waves
+// w = tw + NW g (tile tw, group g); each group's NW partial tiles are reduced separately.
+// Six-tile grids then run 12-wave blocks (three waves on every SIMD) instead of 6-wave ones.
+template <int NW, int NLR, int NCG, int PREC, int NF4 = 0, int NG = 1>
+__global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
     const double4 *__restrict__ pm, const int *__restrict__ i0tab, const double2 *__restrict__ z0tab,
@@ -451,24 +454,28 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     int spec_ltop, int mmax, int mres, int nlat, int nlat_2, int ltr, int imlim, int nfields, int vector_mode,
     double mpos_scale, int lat_weight, int cf_lds, int tile_base) {
     static_assert(NF4 == 0 || (PREC == 0 && NCG == 1 && NF4 <= 3), "4x4x4 path: FP64, one group of <= 3 fields");
+    static_assert(NG == 1 || (NF4 == 0 && NLR == 1), "several field groups per block: one tile per wave");
     constexpr int NA = NF4 ? NF4 : NCG;           // operand / accumulator sets
     constexpr int NCOL = NF4 ? 4 * NF4 : 16 * NCG;
-    __shared__ double ytile[NW][16][65];
+    constexpr int NT = 64 * NW * NG;              // threads per block
+    __shared__ double ytile[NW * NG][16][65];
     // cf_lds: the order's recurrence coefficients (alpha_i, beta_i), i < nsteps, are copied to
     // LDS once per block (dynamic LDS), so each 16-step group reads them at LDS latency instead
     // of waiting on scalar loads that miss to HBM
     extern __shared__ double2 cfl[];
-    __shared__ double red[NW][16][NCOL + 1];   // single-buffered: two barriers per block, 2 blocks/CU fit
-    __shared__ double carry[2][NCOL];
-    __shared__ int s_ibeg[NW];
+    __shared__ double red[NG][NW][16][NCOL + 1];   // single-buffered: two barriers per block
+    __shared__ double carry[NG][2][NCOL];
+    __shared__ int s_ibeg[NW * NG];
 
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    // block -> (order, field group), field groups fastest: blocks are dispatched in index
+    const int tw = NG == 1 ? wave : wave % NW, gw = NG == 1 ? 0 : wave / NW;   // tile wave, field group
+    // block -> (order, field groups), field groups fastest: blocks are dispatched in index
     // order, so all field groups of the heavy low orders start first
     const int ngr = gridDim.x / (mmax + 1);
     const int im = blockIdx.x / ngr;
-    const int fbase = (blockIdx.x - im * ngr) * 4 * NCG;
+    const int fblk = (blockIdx.x - im * ngr) * 4 * NCG * NG;   // first field of the block
+    const int fbase = fblk + gw * 4 * NCG;                    // first field of this wave's group
 
     const OrderInfo o = order_info(im, lmax_spec, mmax, mres, (im <= imlim) ? ltr : -1, vector_mode);
     const int nsteps = (im <= imlim) ? o.nsteps : 0;
@@ -486,7 +493,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     if (NCF > 0 && cf_lds) {
 #pragma unroll
         for (int q = 0; q < NCF; ++q) {
-            const int e = threadIdx.x + q * 64 * NW;
+            const int e = threadIdx.x + q * NT;
             cfr[q] = (e < nsteps) ? ab[sofs + e] : make_double2(0.0, 0.0);
         }
     }
@@ -498,7 +505,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     int my_ibeg = nsteps, my_imax = 0;
 #pragma unroll
     for (int r = 0; r < NLR; ++r) {
-        const int k = (tile_base + wave + NW * r) * 64 + lane;
+        const int k = (tile_base + tw + NW * r) * 64 + lane;
         const bool kv = (k < nlat_2);
         const int kk = kv ? k : 0;
         x2v[r] = x2k[kk];
@@ -527,7 +534,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
             const double2 *Ff = F + (size_t)(fv ? f : 0) * four_dist + (size_t)im * nlat;
 #pragma unroll
             for (int kg = 0; kg < 16; ++kg) {
-                const int kl = (tile_base + wave + NW * r) * 64 + 4 * kg + (lane >> 4);
+                const int kl = (tile_base + tw + NW * r) * 64 + 4 * kg + (lane >> 4);
                 const bool kv = fv && kl < nlat_2;
                 if constexpr (NLR >= 3) {
                     // several tiles per wave: one K-group at a time (loads hoisted together
@@ -559,18 +566,18 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     }
     my_imax = __builtin_amdgcn_readfirstlane(my_imax);
     if (lane == 0) s_ibeg[wave] = my_ibeg;
-    if (threadIdx.x < 2 * NCOL) (&carry[0][0])[threadIdx.x] = 0.0;
+    if (threadIdx.x < 2 * NCOL * NG) (&carry[0][0][0])[threadIdx.x] = 0.0;
     if (cf_lds) {
 #pragma unroll
         for (int q = 0; q < NCF; ++q) {
-            const int e = threadIdx.x + q * 64 * NW;
+            const int e = threadIdx.x + q * NT;
             if (e < nsteps) cfl[e] = cfr[q];
         }
-        for (int e = threadIdx.x + NCF * 64 * NW; e < nsteps; e += 64 * NW) cfl[e] = ab[sofs + e];   // rest, if any
+        for (int e = threadIdx.x + NCF * NT; e < nsteps; e += NT) cfl[e] = ab[sofs + e];   // rest, if any
     }
     __syncthreads();
     int ibeg = nsteps;
-    for (int w = 0; w < NW; ++w) ibeg = min(ibeg, s_ibeg[w]);
+    for (int w = 0; w < NW * NG; ++w) ibeg = min(ibeg, s_ibeg[w]);
     ibeg = __builtin_amdgcn_readfirstlane(ibeg) & ~15;   // block-uniform, scalar from here on
 #pragma unroll
     for (int r = 0; r < NLR; ++r) {
@@ -583,9 +590,10 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
     const int nsteps_out = (spec_ltop >= o.mfun) ? (spec_ltop - o.mfun) / 2 + 1 : 0;
     double2 *Qbase = Q + o.spec_base;
 
-    // post-mix work split: thread handles elements e = threadIdx.x + j*64*NW (j < NE) of the
-    // 16 x NCOL block; their coefficient rows are prefetched before the MFMA phase.
-    constexpr int NE = (16 * NCOL + 64 * NW - 1) / (64 * NW);
+    // post-mix work split: thread handles elements e = threadIdx.x + j*NT (j < NE) of the
+    // 16 x NCOL block, each for all NG field groups (the same coefficient row); the
+    // coefficient rows are prefetched before the MFMA phase.
+    constexpr int NE = (16 * NCOL + NT - 1) / NT;
     auto cf_at = [&](auto from_lds, int idx) {
         idx = min(idx, nsteps - 1);
         if constexpr (decltype(from_lds)::value) return cfl[idx];   // uniform: LDS broadcast
@@ -626,7 +634,7 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
         double pcx[NE], pcz[NE], pcp[NE];
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
-            const int e = threadIdx.x + j * 64 * NW;
+            const int e = threadIdx.x + j * NT;
             const int row = min(e / NCOL, 15);
             const int ii = min(i + row, nsteps - 1);
             const double4 cf4 = pmm[ii];
@@ -672,80 +680,84 @@ __global__ __launch_bounds__(64 * NW) void legendre_anal_kernel(
         if constexpr (NF4 > 0) {
             // 4x4x4 D: step 4 ((lane >> 2) & 3) + (lane >> 4), column lane & 3 of field c
 #pragma unroll
-            for (int c = 0; c < NA; ++c) red[wave][4 * ((lane >> 2) & 3) + (lane >> 4)][4 * c + (lane & 3)] = acc[c];
+            for (int c = 0; c < NA; ++c) red[gw][tw][4 * ((lane >> 2) & 3) + (lane >> 4)][4 * c + (lane & 3)] = acc[c];
         } else {
             // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
 #pragma unroll
             for (int c = 0; c < NCG; ++c)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) red[wave][C::d_row(j, lane)][16 * c + (lane & 15)] = acc[c][j];
+                for (int j = 0; j < 4; ++j) red[gw][tw][C::d_row(j, lane)][16 * c + (lane & 15)] = acc[c][j];
         }
         __syncthreads();
-        // sum over waves + post-mix
+        // sum over each group's waves + post-mix
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
-            const int e = threadIdx.x + j * 64 * NW;
+            const int e = threadIdx.x + j * NT;
             if (e >= 16 * NCOL) break;
             const int row = e / NCOL, col = e - row * NCOL;
             const int comp = col & 3;
-            const int f = fbase + (col >> 2);
-            double sum = 0.0, sprev = 0.0;
-#pragma unroll
-            for (int w = 0; w < NW; ++w) sum += red[w][row][col];
-            if (row == 15 && comp < 2) carry[pb][col] = sum;
             const int ii = i + row;
-            if (f >= nfields || ii >= nsteps_out) continue;
-            const int le = o.mfun + 2 * ii;
-            double v = 0.0;
-            if (ii < nsteps) {
-                if (comp < 2) {
-                    if (row > 0) {
 #pragma unroll
-                        for (int w = 0; w < NW; ++w) sprev += red[w][row - 1][col];
+            for (int g = 0; g < NG; ++g) {
+                const int f = fblk + g * 4 * NCG + (col >> 2);
+                double sum = 0.0, sprev = 0.0;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) sum += red[g][w][row][col];
+                if (row == 15 && comp < 2) carry[g][pb][col] = sum;
+                if (f >= nfields || ii >= nsteps_out) continue;
+                const int le = o.mfun + 2 * ii;
+                double v = 0.0;
+                if (ii < nsteps) {
+                    if (comp < 2) {
+                        if (row > 0) {
+#pragma unroll
+                            for (int w = 0; w < NW; ++w) sprev += red[g][w][row - 1][col];
+                        } else {
+                            sprev = carry[g][pb ^ 1][col];
+                        }
+                        v = pcx[j] * sum + pcp[j] * sprev;
                     } else {
-                        sprev = carry[pb ^ 1][col];
+                        v = pcz[j] * sum;
                     }
-                    v = pcx[j] * sum + pcp[j] * sprev;
-                } else {
-                    v = pcz[j] * sum;
                 }
-            }
-            const int ld = (comp < 2) ? le : le + 1;
-            if (ld <= spec_ltop) {
-                double out = (ld <= ltr) ? v * qscale : 0.0;
-                if (o.real_only && (comp & 1)) out = 0.0;
-                reinterpret_cast<double *>(&Qbase[(size_t)f * spec_dist + ld])[comp & 1] = out;
+                const int ld = (comp < 2) ? le : le + 1;
+                if (ld <= spec_ltop) {
+                    double out = (ld <= ltr) ? v * qscale : 0.0;
+                    if (o.real_only && (comp & 1)) out = 0.0;
+                    reinterpret_cast<double *>(&Qbase[(size_t)f * spec_dist + ld])[comp & 1] = out;
+                }
             }
         }
     }
     // degrees not reached by the step loop: before ibeg (negligible functions) and after nsteps
-    for (int ii = threadIdx.x; ii < nsteps_out; ii += 64 * NW) {
+    for (int ii = threadIdx.x; ii < nsteps_out; ii += NT) {
         if (ii >= ibeg && ii < ibeg + ((nsteps - ibeg + 15) & ~15)) continue;
         const int le = o.mfun + 2 * ii;
-        for (int f = fbase; f < min(nfields, fbase + 4 * NCG); ++f) {
+        for (int f = fblk; f < min(nfields, fblk + 4 * NCG * NG); ++f) {
             double2 *Qf = Qbase + (size_t)f * spec_dist;
             if (le <= spec_ltop) Qf[le] = make_double2(0.0, 0.0);
             if (le + 1 <= spec_ltop) Qf[le + 1] = make_double2(0.0, 0.0);
         }
     }
     if (vector_mode && im == 0 && threadIdx.x == 0) {   // degree 0 of the vector m=0 spectra
-        for (int f = fbase; f < min(nfields, fbase + 4 * NCG); ++f) Qbase[(size_t)f * spec_dist] = make_double2(0.0, 0.0);
+        for (int f = fblk; f < min(nfields, fblk + 4 * NCG * NG); ++f) Qbase[(size_t)f * spec_dist] = make_double2(0.0, 0.0);
     }
 }
 
-template <int NW, int NLR, int NCG, int PREC, int NF4 = 0>
+template <int NW, int NLR, int NCG, int PREC, int NF4 = 0, int NG = 1>
 void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                  int spec_ltop, hipStream_t s, int tile_base = 0) {
     const DeviceTables *t = a.t;
-    dim3 grid((a.mmax + 1) * ((a.nfields + 4 * NCG - 1) / (4 * NCG)));
+    dim3 grid((a.mmax + 1) * ((a.nfields + 4 * NCG * NG - 1) / (4 * NCG * NG)));
     // static LDS of the kernel, and the coefficient table if it fits beside it in 160 KB
     constexpr size_t NCOL = NF4 ? 4 * NF4 : 16 * NCG;
-    constexpr size_t static_lds = sizeof(double) * (NW * 16 * 65 + NW * 16 * (NCOL + 1) + 2 * NCOL) + sizeof(int) * NW;
+    constexpr size_t static_lds =
+        sizeof(double) * NG * (NW * 16 * 65 + NW * 16 * (NCOL + 1) + 2 * NCOL) + sizeof(int) * NW * NG;
     const size_t cf_bytes = sizeof(double2) * (size_t)(a.ltr / 2 + 2);
     const int cf_lds = (static_lds + cf_bytes + 1024 <= 160 * 1024) ? 1 : 0;
     const size_t dyn = cf_lds ? cf_bytes : 0;
-    ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC, NF4>, dyn);
-    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC, NF4>), grid, dim3(64 * NW), dyn, s, t->x, t->x2, t->st,
+    ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC, NF4, NG>, dyn);
+    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC, NF4, NG>), grid, dim3(64 * NW * NG), dyn, s, t->x, t->x2, t->st,
                        t->wq, t->step_ofs, t->ab, t->pm, a.vector_mode ? t->i0v : t->i0,
                        a.vector_mode ? t->z0v : t->z0, F, four_dist, Q, spec_dist,
                        spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
@@ -791,6 +803,13 @@ void anal_launch(const LegendreArgs &a, const double2 *F, long four_dist, double
     if (ncg_want == 2 && nlr <= 3) wide = (a.nfields > 4);
     // one or two fields (FP64): 4x4x4 blocks, no idle matrix-core columns
     const int nf4 = (PREC == 0 && a.nfields <= 2 && !synth_force16()) ? a.nfields : 0;
+    // six one-tile waves and more than one 8-field group: two groups per block, 12 waves (three
+    // on every SIMD instead of 1.5); SHTNS_AMD_ANAL_NG=1 keeps the 6-wave blocks
+    static const int ng_env = getenv("SHTNS_AMD_ANAL_NG") ? atoi(getenv("SHTNS_AMD_ANAL_NG")) : 0;
+    if (nf4 == 0 && nw == 6 && nlr == 1 && wide && a.nfields > 8 && ng_env != 1) {
+        launch_anal_t<6, 1, 2, PREC, 0, 2>(a, F, four_dist, Q, spec_dist, spec_ltop, s);
+        return;
+    }
 #define ANAL_CASE(W, R)                                                                              \
     if (nw == W && nlr <= R) {                                                                       \
         if constexpr (PREC == 0) {                                                                   \
