@@ -1376,6 +1376,29 @@ static bool ensure_i8_tables(shtns_cfg c) {
     return true;
 }
 
+// Interleaved complex spatial fields as the source of a batch of real fields: real field k is
+// part k % 2 (real, imaginary) of complex field k / 2, at z + (k / 2) * zdist complex
+// elements.  The analysis FFTs then read the complex arrays directly.
+struct SpatPairs {
+    double *z;
+    long zdist;
+};
+
+// longitude FFTs of analysis: V (nf real fields starting at real field k0) -> F
+static void fft_anal_from(shtns_cfg c, int imlim, int nf, int k0, const double *V, long spat_dist, const SpatPairs *zp,
+                          double2 *F, long four_dist, hipStream_t s, int kind = 0) {
+    long sp, st;
+    spat_strides(c, sp, st);
+    if (!zp) {
+        launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf, V + (size_t)k0 * spat_dist, spat_dist, F,
+                        four_dist, sp, st, s, 0, -1, kind);
+        return;
+    }
+    for (int part = 0; part < 2; ++part)
+        launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf / 2, zp->z + (size_t)(k0 / 2) * 2 * zp->zdist + part,
+                        2 * zp->zdist, F + (size_t)part * four_dist, 2 * four_dist, sp, st, s, 0, -1, kind, 2);
+}
+
 // Scratch bytes of one synth_core call (Fourier rows F, then the pre-mixed coefficients).
 static size_t synth_scratch_bytes(shtns_cfg c, int nfields) {
     const size_t fbytes = (size_t)nfields * (c->mmax + 1) * c->nlat * sizeof(double2);
@@ -1497,16 +1520,19 @@ static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nf
 // V is double (vf == nullptr) or float (vf != nullptr, vstage as in synth_core: when set,
 // vf is first converted into it).
 static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
-                         double mpos_scale, int lat_weight, int prec, const float *vf, double *vstage);
+                         double mpos_scale, int lat_weight, int prec, const float *vf, double *vstage,
+                         const SpatPairs *zp);
+// zp: the spatial input is read from interleaved complex fields instead of V (nfields even)
 static void anal_core(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
                       double mpos_scale, int lat_weight, int prec, const float *vf = nullptr,
-                      double *vstage = nullptr) {
+                      double *vstage = nullptr, const SpatPairs *zp = nullptr) {
     if (nfields <= 0) return;
-    anal_core_on(c, V, Q, ltr, nfields, spat_dist, spec_dist, mpos_scale, lat_weight, prec, vf, vstage);
+    anal_core_on(c, V, Q, ltr, nfields, spat_dist, spec_dist, mpos_scale, lat_weight, prec, vf, vstage, zp);
     scale_output(c, Q, spec_dist, nfields, lat_weight /* analysis: 1 / nrm; adjoint of the synthesis: nrm */);
 }
 static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfields, long spat_dist, long spec_dist,
-                         double mpos_scale, int lat_weight, int prec, const float *vf, double *vstage) {
+                         double mpos_scale, int lat_weight, int prec, const float *vf, double *vstage,
+                         const SpatPairs *zp) {
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
@@ -1529,8 +1555,7 @@ static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfi
         {
             ProfScope ps(c, PK_FFT_ANAL);
             if (!vf) {
-                launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st,
-                                S, 0, -1, c->tune_fft_anal);
+                fft_anal_from(c, imlim, nfields, 0, V, spat_dist, zp, F, four_dist, S, c->tune_fft_anal);
             } else if (!vstage) {
                 launch_fft_anal_wave(&c->d, c->nlat, c->nphi / 2, c->mmax, imlim, nfields, vf, spat_dist, F,
                                      four_dist, sp, st, S, 0, c->nlat);
@@ -1558,8 +1583,7 @@ static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfi
         const int f0 = ch * chunk, nf = std::min(chunk, nfields - f0);
         {
             ProfScope ps(c, PK_FFT_ANAL, A, true);
-            launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf, V + (size_t)f0 * spat_dist, spat_dist,
-                            F + (size_t)f0 * four_dist, four_dist, sp, st, A);
+            fft_anal_from(c, imlim, nf, f0, V, spat_dist, zp, F + (size_t)f0 * four_dist, four_dist, A);
         }
         hipEvent_t ev = pool_event(c, 1 + ch);
         HIP_OK(hipEventRecord(ev, A));
@@ -1611,8 +1635,9 @@ static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, d
     HIP_OK(hipGetLastError());
 }
 
+// zt, zpp: the two components are read from interleaved complex fields instead of Vt, Vp
 static void vanal_core(shtns_cfg c, const double *Vt, const double *Vp, cplx *S, cplx *T, int ltr, int nfields,
-                       long spat_dist, long spec_dist) {
+                       long spat_dist, long spec_dist, const SpatPairs *zt = nullptr, const SpatPairs *zpp = nullptr) {
     if (nfields <= 0) return;
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
@@ -1632,12 +1657,8 @@ static void vanal_core(shtns_cfg c, const double *Vt, const double *Vp, cplx *S,
     double2 *VW = (double2 *)(base + fbytes);
     a.part = pbytes ? (double2 *)(base + fbytes + vbytes) : nullptr;
     a.part_dist = vw_dist;
-    long sp, st;
-    spat_strides(c, sp, st);
-    launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, Vt, spat_dist, F, 2 * four_dist, sp, st,
-                    c->comp_stream);
-    launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, Vp, spat_dist, F + four_dist, 2 * four_dist, sp,
-                    st, c->comp_stream);
+    fft_anal_from(c, imlim, nfields, 0, Vt, spat_dist, zt, F, 2 * four_dist, c->comp_stream);
+    fft_anal_from(c, imlim, nfields, 0, Vp, spat_dist, zpp, F + four_dist, 2 * four_dist, c->comp_stream);
     launch_legendre_anal(a, F, four_dist, VW, vw_dist, c->lmax + 1, c->comp_stream);
     launch_vw_to_sphtor(VW, vw_dist, reinterpret_cast<double2 *>(S), reinterpret_cast<double2 *>(T), spec_dist,
                         c->lmax, c->mmax, c->mres, ltr, imlim, nfields, c->comp_stream);
@@ -1943,7 +1964,8 @@ static bool cplx_batch_ok(shtns_cfg c, const char *who, int nf, long spec_dist, 
 }
 /* Batches of complex fields: field f at alm + f*spec_dist and z + f*spat_dist (complex
    elements).  The fields' real and imaginary parts go through the real kernels as one batch
-   of 2 nf real fields. */
+   of 2 nf real fields.  The synthesis writes them to a staging batch and interleaves it; the
+   analysis FFTs read the interleaved complex arrays directly (one launch per part). */
 extern "C" void cu_SH_to_spat_cplx_batch(shtns_cfg c, cplx *alm, cplx *z, int nf, long spec_dist, long spat_dist) {
     clear_error();
     if (!cplx_batch_ok(c, "cu_SH_to_spat_cplx", nf, spec_dist, spat_dist)) return;
@@ -1962,13 +1984,12 @@ extern "C" void cu_SH_to_spat_cplx_batch(shtns_cfg c, cplx *alm, cplx *z, int nf
 extern "C" void cu_spat_cplx_to_SH_batch(shtns_cfg c, cplx *z, cplx *alm, int nf, long spat_dist, long spec_dist) {
     clear_error();
     if (!cplx_batch_ok(c, "cu_spat_cplx_to_SH", nf, spec_dist, spat_dist)) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
-    char *base = (char *)xscratch(c, qb + vb);
-    if (!base) return;
-    double2 *XY = (double2 *)base;
-    double *xy = (double *)(base + qb);
-    launch_deinterleave(reinterpret_cast<const double2 *>(z), spat_dist, xy, c->nspat, nf, c->comp_stream);
-    anal_core(c, xy, reinterpret_cast<cplx *>(XY), c->lmax, 2 * nf, c->nspat, c->nlm, 1.0, 1, prec_f64(c));
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf;
+    double2 *XY = (double2 *)xscratch(c, qb);
+    if (!XY) return;
+    const SpatPairs zp{reinterpret_cast<double *>(z), spat_dist};
+    anal_core(c, nullptr, reinterpret_cast<cplx *>(XY), c->lmax, 2 * nf, c->nspat, c->nlm, 1.0, 1, prec_f64(c),
+              nullptr, nullptr, &zp);
     if (g_last_err) return;
     launch_cplx_merge(XY, c->nlm, reinterpret_cast<double2 *>(alm), spec_dist, c->lmax, c->mmax, nf, c->comp_stream);
     HIP_OK(hipGetLastError());
@@ -2005,16 +2026,14 @@ extern "C" void cu_spat_cplx_to_SHsphtor_batch(shtns_cfg c, cplx *zt, cplx *zp, 
                                                long spat_dist, long spec_dist) {
     clear_error();
     if (!cplx_batch_ok(c, "cu_spat_cplx_to_SHsphtor", nf, spec_dist, spat_dist)) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
-    char *base = (char *)xscratch(c, 2 * qb + 2 * vb);
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf;
+    char *base = (char *)xscratch(c, 2 * qb);
     if (!base) return;
     double2 *SS = (double2 *)base, *TT = (double2 *)(base + qb);
-    double *VT = (double *)(base + 2 * qb), *VP = (double *)(base + 2 * qb + vb);
     const hipStream_t s = c->comp_stream;
-    launch_deinterleave(reinterpret_cast<const double2 *>(zt), spat_dist, VT, c->nspat, nf, s);
-    launch_deinterleave(reinterpret_cast<const double2 *>(zp), spat_dist, VP, c->nspat, nf, s);
-    vanal_core(c, VT, VP, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), c->lmax, 2 * nf, c->nspat,
-               c->nlm);
+    const SpatPairs pt{reinterpret_cast<double *>(zt), spat_dist}, pp{reinterpret_cast<double *>(zp), spat_dist};
+    vanal_core(c, nullptr, nullptr, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), c->lmax, 2 * nf,
+               c->nspat, c->nlm, &pt, &pp);
     if (g_last_err) return;
     launch_cplx_merge(SS, c->nlm, reinterpret_cast<double2 *>(slm), spec_dist, c->lmax, c->mmax, nf, s);
     launch_cplx_merge(TT, c->nlm, reinterpret_cast<double2 *>(tlm), spec_dist, c->lmax, c->mmax, nf, s);

@@ -195,13 +195,15 @@ void launch_legendre_anal(const LegendreArgs &a, const double2 *F, long four_dis
 // (nlat_chunk < 0: all latitudes).
 // kind: 0 automatic, 1 register FFT, 2 LDS-staged FFT (where a kernel for the length exists),
 // 3 / 4 block-cooperative register FFT with 256 / 128 threads per block, 5 / 6 the same with
-// 8-byte LDS staging and 256 / 512 threads
+// 8-byte LDS staging and 256 / 512 threads.
+// es (analysis): element stride of V in doubles (2: one part of interleaved complex values);
+// the kernel choice follows the layout's strides sp, st, and the kernels step sp * es, st * es.
 void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                       const double2 *F, long four_dist, double *V, long spat_dist, long sp, long st, hipStream_t s,
                       int lat_base = 0, int nlat_chunk = -1, int kind = 0);
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                      const double *V, long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s,
-                     int lat_base = 0, int nlat_chunk = -1, int kind = 0);
+                     int lat_base = 0, int nlat_chunk = -1, int kind = 0, int es = 1);
 // Register-resident FFT kernels (sht_fft.hip) for M = nphi/2 in {16, 24} x 2^k up to 6144,
 // latitudes [lat_base, lat_end); return false (nothing launched) for other M.
 bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
@@ -234,12 +236,12 @@ void ensure_lds_limit(const void *kern, size_t lds);
 void launch_convert(const float *src, long src_dist, double *dst, long dst_dist, long n, int nfields, hipStream_t s);
 void launch_convert(const double *src, long src_dist, float *dst, long dst_dist, long n, int nfields, hipStream_t s);
 // complex-valued fields: spectra split into / merged from the real and imaginary parts'
-// real-field spectra, and the spatial interleave of the two parts
-// (nf fields; field f's real-part / imaginary-part real fields are entries 2f and 2f+1 of a batch)
+// real-field spectra (nf fields; field f's real-part / imaginary-part real fields are entries
+// 2f and 2f+1 of a batch), and the spatial interleave of the synthesis (the analysis FFTs
+// read the interleaved complex arrays directly)
 void launch_cplx_split(const double2 *a, long a_dist, double2 *XY, long nlm, int lmax, int mmax, int nf, hipStream_t s);
 void launch_cplx_merge(const double2 *XY, long nlm, double2 *a, long a_dist, int lmax, int mmax, int nf, hipStream_t s);
 void launch_interleave(const double *xy, double2 *z, long z_dist, long n, int nf, hipStream_t s);
-void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int nf, hipStream_t s);
 // Legendre transforms at one order (no FFT): table of P_lm at every latitude, sum over degrees,
 // weighted sum over latitudes
 // sliced int8 synthesis (sht_legendre_i8.hip): row-exponent table, scratch sizes, launch

@@ -323,21 +323,14 @@ __global__ void cplx_merge_kernel(const double2 *__restrict__ XY, long nlm, doub
     }
 }
 
-// z_f[i] = (x_f[i], y_f[i]) and back; x_f = xy + 2 f n, y_f = x_f + n; z_f = z + f z_dist
+// z_f[i] = (x_f[i], y_f[i]); x_f = xy + 2 f n, y_f = x_f + n; z_f = z + f z_dist.  (The
+// synthesis writes the two parts into a staging batch and interleaves them here: FFT stores of
+// one part straight into the interleaved array would write half cache lines twice.)
 __global__ void interleave_kernel(const double *__restrict__ xy, double2 *__restrict__ z, long z_dist, long n) {
     const double *x = xy + (size_t)blockIdx.y * 2 * n;
     double2 *zf = z + (size_t)blockIdx.y * z_dist;
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
         zf[i] = make_double2(x[i], x[n + i]);
-}
-__global__ void deinterleave_kernel(const double2 *__restrict__ z, long z_dist, double *__restrict__ xy, long n) {
-    double *x = xy + (size_t)blockIdx.y * 2 * n;
-    const double2 *zf = z + (size_t)blockIdx.y * z_dist;
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const double2 v = zf[i];
-        x[i] = v.x;
-        x[n + i] = v.y;
-    }
 }
 
 void launch_cplx_split(const double2 *a, long a_dist, double2 *XY, long nlm, int lmax, int mmax, int nf, hipStream_t s) {
@@ -352,11 +345,6 @@ void launch_interleave(const double *xy, double2 *z, long z_dist, long n, int nf
     if (n <= 0 || nf <= 0) return;
     const int nb = (int)std::min<long>((n + 255) / 256, 4096);
     hipLaunchKernelGGL(interleave_kernel, dim3(nb, nf), dim3(256), 0, s, xy, z, z_dist, n);
-}
-void launch_deinterleave(const double2 *z, long z_dist, double *xy, long n, int nf, hipStream_t s) {
-    if (n <= 0 || nf <= 0) return;
-    const int nb = (int)std::min<long>((n + 255) / 256, 4096);
-    hipLaunchKernelGGL(deinterleave_kernel, dim3(nb, nf), dim3(256), 0, s, z, z_dist, xy, n);
 }
 
 // ---- Legendre transforms at one order m (no FFT) ----
@@ -935,12 +923,14 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
 
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields, const double *V,
                      long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
-                     int nlat_chunk, int kind) {
+                     int nlat_chunk, int kind, int es) {
     const int M = nphi / 2;
     if (nlat_chunk < 0) { lat_base = 0; nlat_chunk = nlat; }
     const int lat_end = lat_base + nlat_chunk;
     static const bool generic = getenv("SHTNS_AMD_FFT_GENERIC") != nullptr;
     const int choice = generic ? 2 : fft_choice(M, sp, kind);
+    sp *= es;
+    st *= es;
     if (choice >= 3 && choice <= 6 &&
         launch_fft_anal_coop(t, nlat, M, mmax, imlim, nfields, V, spat_dist, F, four_dist, sp, st, s, lat_base, lat_end,
                              coop_threads(choice)))
