@@ -641,15 +641,26 @@ struct CoopSGeom {
     static constexpr size_t lds_bytes = (size_t)T * STR * sizeof(double);
 };
 
+// fpart > 0: the two parts (real, imaginary) of interleaved complex fields.  Field y's part p
+// reads F + y four_dist + p fpart and writes V + y spat_dist + p (strides sp, stt already
+// those of the doubles of the complex array).  gridDim.x is twice the latitude groups, and
+// the part is bit 3 of the block index: blocks b and b + 8 run on the same XCD, one after
+// the other, and write the two halves of the same cache lines, which merge in that L2.
 template <int M, int NTB>
 __global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict__ F, long four_dist,
                                                        double *__restrict__ V, long spat_dist,
                                                        const double2 *__restrict__ roots, int nlat, int imlim,
-                                                       long sp, long stt, int lat_base, int lat_end) {
+                                                       long sp, long stt, int lat_base, int lat_end, long fpart) {
     using CG = CoopSGeom<M, NTB>;
     constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, STR = CG::STR, ROWS = CG::ROWS;
     extern __shared__ double ldsd[];
-    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
+    int bx = blockIdx.x, nbx = gridDim.x, part = 0;
+    if (fpart > 0) {
+        part = (bx >> 3) & 1;
+        bx = ((bx >> 4) << 3) | (bx & 7);
+        nbx >>= 1;
+    }
+    const int lat0 = lat_base + xcd_group_w(bx, nbx) * T;
     const int tl = threadIdx.x % T, row = threadIdx.x / T;
     const bool live = lat0 + tl < lat_end;
     double *mine = ldsd + tl * STR;                // staging rows of latitude tl
@@ -658,7 +669,7 @@ __global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict
     // X_k of the block's latitudes, 16-byte loads with the latitude fastest
     double2 xr[R];
     {
-        const double2 *Ff = F + (size_t)blockIdx.y * four_dist;
+        const double2 *Ff = F + (size_t)blockIdx.y * four_dist + (size_t)part * fpart;
         const unsigned off = 16u * (unsigned)(lat0 + tl + row * nlat), step = 16u * (unsigned)(ROWS * nlat);
         sfor<R>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
@@ -698,7 +709,7 @@ __global__ __launch_bounds__(NTB) void fft_synth_coops(const double2 *__restrict
     // z_n out: real parts to rows 2n, then imaginary parts to rows 2n+1
     const unsigned off = 8u * (unsigned)((live ? lat0 + tl : 0) * stt + 2 * row * sp);
     const unsigned step = 8u * (unsigned)(2 * ROWS * sp), odd = 8u * (unsigned)sp;
-    double *Vf = V + (size_t)blockIdx.y * spat_dist;
+    double *Vf = V + (size_t)blockIdx.y * spat_dist + part;
     sfor<R>([&](auto jc) { seq[slot_at<LQ>(l, decltype(jc)::value)] = v[decltype(jc)::value].x; });
     __syncthreads();
     if (live)
@@ -921,15 +932,19 @@ bool anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int 
 
 template <int NTB>
 bool synth_coops(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F, long four_dist,
-                 double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+                 double *V, long spat_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end,
+                 long fpart = 0, bool dry = false) {
 #define SYN_COOPS(MM)                                                                                         \
     if (M == MM) {                                                                                            \
         using CG = CoopSGeom<MM, NTB>;                                                                        \
         static_assert(CG::LQ <= 64 && CG::T >= 2 && CG::T * CG::LQ == CG::NT, "cooperative FFT shape");      \
         lds_limit(fft_synth_coops<MM, NTB>, CG::lds_bytes);                                                   \
         const int nb = (lat_end - lat_base + CG::T - 1) / CG::T;                                             \
-        hipLaunchKernelGGL((fft_synth_coops<MM, NTB>), dim3(nb, nfields), dim3(CG::NT), CG::lds_bytes, s, F,  \
-                           four_dist, V, spat_dist, t->roots, nlat, imlim, sp, st, lat_base, lat_end);        \
+        if (fpart > 0 && nb % 8) return false;   /* the part / XCD decoding needs whole groups of 8 */        \
+        if (dry) return true;                                                                                 \
+        hipLaunchKernelGGL((fft_synth_coops<MM, NTB>), dim3(fpart > 0 ? 2 * nb : nb, nfields), dim3(CG::NT),  \
+                           CG::lds_bytes, s, F, four_dist, V, spat_dist, t->roots, nlat, imlim, sp, st,       \
+                           lat_base, lat_end, fpart);                                                         \
         return true;                                                                                          \
     }
     COOP_FFT_SIZES(SYN_COOPS)
@@ -974,6 +989,17 @@ bool launch_fft_synth_coop(const DeviceTables *t, int nlat, int M, int imlim, in
     if (nthreads == 128)
         return synth_coop<128>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
     return synth_coop<256>(t, nlat, M, imlim, nfields, F, four_dist, V, spat_dist, sp, st, s, lat_base, lat_end);
+}
+
+bool launch_fft_synth_coops_pairs(const DeviceTables *t, int nlat, int M, int imlim, int nf, const double2 *F,
+                                  long fdist, long fpart, double *V, long spat_dist, long sp, long st, hipStream_t s,
+                                  int nthreads, bool dry) {
+    if (fpart <= 0 || !coop_offsets_fit(nlat, M, imlim, sp, st)) return false;
+    if (nthreads == -512)
+        return synth_coops<512>(t, nlat, M, imlim, nf, F, fdist, V, spat_dist, sp, st, s, 0, nlat, fpart, dry);
+    if (nthreads == -256)
+        return synth_coops<256>(t, nlat, M, imlim, nf, F, fdist, V, spat_dist, sp, st, s, 0, nlat, fpart, dry);
+    return false;
 }
 
 bool launch_fft_anal_coop(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,

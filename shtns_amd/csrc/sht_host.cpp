@@ -1376,13 +1376,35 @@ static bool ensure_i8_tables(shtns_cfg c) {
     return true;
 }
 
-// Interleaved complex spatial fields as the source of a batch of real fields: real field k is
-// part k % 2 (real, imaginary) of complex field k / 2, at z + (k / 2) * zdist complex
-// elements.  The analysis FFTs then read the complex arrays directly.
+// Interleaved complex spatial fields as the target / source of a batch of real fields: real
+// field k is part k % 2 (real, imaginary) of complex field k / 2, at z + (k / 2) * zdist
+// complex elements.  The FFTs then write / read the complex arrays directly.
 struct SpatPairs {
     double *z;
     long zdist;
 };
+
+// whether the synthesis FFTs of this configuration can write interleaved pairs directly
+static bool synth_pairs_ok(shtns_cfg c) {
+    long sp, st;
+    spat_strides(c, sp, st);
+    return !overlap_enabled() && launch_fft_synth_pairs(&c->d, c->nlat, c->nphi, c->mmax, 1, nullptr, 1, 1, nullptr, 1,
+                                                        sp, st, nullptr, c->tune_fft_synth, true);
+}
+// longitude FFTs of synthesis: F (nf real fields, field distance fdist) -> V, or into
+// interleaved pairs (then synth_pairs_ok(c) holds)
+static void fft_synth_to(shtns_cfg c, int imlim, int nf, const double2 *F, long fdist, double *V, long spat_dist,
+                         const SpatPairs *zp, hipStream_t s, int kind = 0) {
+    long sp, st;
+    spat_strides(c, sp, st);
+    if (!zp) {
+        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf, F, fdist, V, spat_dist, sp, st, s, 0, -1, kind);
+        return;
+    }
+    if (!launch_fft_synth_pairs(&c->d, c->nlat, c->nphi, imlim, nf / 2, F, 2 * fdist, fdist, zp->z, 2 * zp->zdist, sp,
+                                st, s, kind))
+        set_error(2, "synthesis into interleaved complex fields: no kernel for this size");
+}
 
 // longitude FFTs of analysis: V (nf real fields starting at real field k0) -> F
 static void fft_anal_from(shtns_cfg c, int imlim, int nf, int k0, const double *V, long spat_dist, const SpatPairs *zp,
@@ -1409,10 +1431,13 @@ static size_t synth_scratch_bytes(shtns_cfg c, int nfields) {
 // bytes is used instead of the configuration's scratch; no stream, event or memory is then
 // created (no profiling, no overlapped chunks).
 static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
-                          double mpos_scale, int lat_weight, int prec, float *vf, double *vstage, void *ext);
+                          double mpos_scale, int lat_weight, int prec, float *vf, double *vstage, void *ext,
+                          const SpatPairs *zp);
+// zp: the spatial output goes to interleaved complex fields instead of V (nfields even,
+// synth_pairs_ok(c))
 static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
                        double mpos_scale, int lat_weight, int prec, float *vf = nullptr, double *vstage = nullptr,
-                       void *ext = nullptr) {
+                       void *ext = nullptr, const SpatPairs *zp = nullptr) {
     if (nfields <= 0) return;
     if (!c->nrm.empty()) {
         if (ext) {
@@ -1422,10 +1447,11 @@ static void synth_core(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfiel
         Q = scaled_input(c, Q, spec_dist, nfields, lat_weight /* adjoint of the analysis: 1 / nrm */, 0);
         if (!Q) return;
     }
-    synth_core_on(c, Q, V, ltr, nfields, spec_dist, spat_dist, mpos_scale, lat_weight, prec, vf, vstage, ext);
+    synth_core_on(c, Q, V, ltr, nfields, spec_dist, spat_dist, mpos_scale, lat_weight, prec, vf, vstage, ext, zp);
 }
 static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nfields, long spec_dist, long spat_dist,
-                          double mpos_scale, int lat_weight, int prec, float *vf, double *vstage, void *ext) {
+                          double mpos_scale, int lat_weight, int prec, float *vf, double *vstage, void *ext,
+                          const SpatPairs *zp) {
     ltr = clamp_ltr(c, ltr);
     const int imlim = imlim_of(c, ltr);
     const long four_dist = (long)(c->mmax + 1) * c->nlat;
@@ -1456,8 +1482,7 @@ static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nf
             launch_legendre_synth_i8(a, c->lmax, F, four_dist, q8, c->nsteps_total, pl, eB, S);
         }
         ProfScope ps(c, PK_FFT_SYNTH, nullptr, false, prof);
-        launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S, 0,
-                         -1, c->tune_fft_synth);
+        fft_synth_to(c, imlim, nfields, F, four_dist, V, spat_dist, zp, S, c->tune_fft_synth);
         HIP_OK(hipGetLastError());
         return;
     }
@@ -1468,7 +1493,7 @@ static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nf
     const int ntiles = (c->nlat_2 + 63) / 64;
     const int chunk_tiles = 4;
     const int nchunks = (ntiles + chunk_tiles - 1) / chunk_tiles;
-    hipStream_t A = (overlap_enabled() && nchunks > 1 && !vf && !ext) ? aux_stream(c) : nullptr;
+    hipStream_t A = (overlap_enabled() && nchunks > 1 && !vf && !ext && !zp) ? aux_stream(c) : nullptr;
     {
         ProfScope ps(c, PK_LEG_SYNTH, nullptr, false, prof);
         launch_premix(a, reinterpret_cast<const double2 *>(Q), spec_dist, c->lmax, qmix, c->nsteps_total, S);
@@ -1480,8 +1505,7 @@ static void synth_core_on(shtns_cfg c, const cplx *Q, double *V, int ltr, int nf
         }
         ProfScope ps(c, PK_FFT_SYNTH, nullptr, false, prof);
         if (!vf) {
-            launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, four_dist, V, spat_dist, sp, st, S, 0,
-                             -1, c->tune_fft_synth);
+            fft_synth_to(c, imlim, nfields, F, four_dist, V, spat_dist, zp, S, c->tune_fft_synth);
         } else if (!vstage) {
             launch_fft_synth_wave(&c->d, c->nlat, c->nphi / 2, imlim, nfields, F, four_dist, vf, spat_dist, sp, st, S,
                                   0, c->nlat);
@@ -1599,8 +1623,10 @@ static void anal_core_on(shtns_cfg c, const double *V, cplx *Q, int ltr, int nfi
     HIP_OK(hipGetLastError());
 }
 
+// zt, zpp: the two components go to interleaved complex fields instead of Vt, Vp (nfields
+// even, synth_pairs_ok(c))
 static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, double *Vp, int ltr, int nfields,
-                        long spec_dist, long spat_dist) {
+                        long spec_dist, long spat_dist, const SpatPairs *zt = nullptr, const SpatPairs *zpp = nullptr) {
     if (nfields <= 0) return;
     if (!c->nrm.empty()) {
         long d2 = spec_dist;
@@ -1626,12 +1652,8 @@ static void vsynth_core(shtns_cfg c, const cplx *S, const cplx *T, double *Vt, d
     a.vector_mode = 1;
     a.prec = prec_f64(c);
     launch_legendre_synth(a, VW, vw_dist, c->lmax + 1, F, four_dist, qmix, c->nsteps_total, c->comp_stream);
-    long sp, st;
-    spat_strides(c, sp, st);
-    launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F, 2 * four_dist, Vt, spat_dist, sp, st,
-                     c->comp_stream);
-    launch_fft_synth(&c->d, c->nlat, c->nphi, c->mmax, imlim, nfields, F + four_dist, 2 * four_dist, Vp, spat_dist, sp,
-                     st, c->comp_stream);
+    fft_synth_to(c, imlim, nfields, F, 2 * four_dist, Vt, spat_dist, zt, c->comp_stream);
+    fft_synth_to(c, imlim, nfields, F + four_dist, 2 * four_dist, Vp, spat_dist, zpp, c->comp_stream);
     HIP_OK(hipGetLastError());
 }
 
@@ -1964,21 +1986,29 @@ static bool cplx_batch_ok(shtns_cfg c, const char *who, int nf, long spec_dist, 
 }
 /* Batches of complex fields: field f at alm + f*spec_dist and z + f*spat_dist (complex
    elements).  The fields' real and imaginary parts go through the real kernels as one batch
-   of 2 nf real fields.  The synthesis writes them to a staging batch and interleaves it; the
-   analysis FFTs read the interleaved complex arrays directly (one launch per part). */
+   of 2 nf real fields.  The FFTs write (read) the interleaved complex arrays directly: the
+   synthesis in one launch per batch when the cooperative FFT kernels serve the size, else
+   through a staging batch and an interleave pass; the analysis in one launch per part. */
 extern "C" void cu_SH_to_spat_cplx_batch(shtns_cfg c, cplx *alm, cplx *z, int nf, long spec_dist, long spat_dist) {
     clear_error();
     if (!cplx_batch_ok(c, "cu_SH_to_spat_cplx", nf, spec_dist, spat_dist)) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
+    const bool direct = synth_pairs_ok(c);
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = direct ? 0 : 2 * sizeof(double) * c->nspat * nf;
     char *base = (char *)xscratch(c, qb + vb);
     if (!base) return;
     double2 *XY = (double2 *)base;
-    double *xy = (double *)(base + qb);
     launch_cplx_split(reinterpret_cast<const double2 *>(alm), spec_dist, XY, c->nlm, c->lmax, c->mmax, nf,
                       c->comp_stream);
-    synth_core(c, reinterpret_cast<const cplx *>(XY), xy, c->lmax, 2 * nf, c->nlm, c->nspat, 1.0, 0, prec_f64(c));
-    if (g_last_err) return;
-    launch_interleave(xy, reinterpret_cast<double2 *>(z), spat_dist, c->nspat, nf, c->comp_stream);
+    if (direct) {
+        const SpatPairs zp{reinterpret_cast<double *>(z), spat_dist};
+        synth_core(c, reinterpret_cast<const cplx *>(XY), nullptr, c->lmax, 2 * nf, c->nlm, c->nspat, 1.0, 0,
+                   prec_f64(c), nullptr, nullptr, nullptr, &zp);
+    } else {
+        double *xy = (double *)(base + qb);
+        synth_core(c, reinterpret_cast<const cplx *>(XY), xy, c->lmax, 2 * nf, c->nlm, c->nspat, 1.0, 0, prec_f64(c));
+        if (g_last_err) return;
+        launch_interleave(xy, reinterpret_cast<double2 *>(z), spat_dist, c->nspat, nf, c->comp_stream);
+    }
     HIP_OK(hipGetLastError());
 }
 extern "C" void cu_spat_cplx_to_SH_batch(shtns_cfg c, cplx *z, cplx *alm, int nf, long spat_dist, long spec_dist) {
@@ -2007,19 +2037,26 @@ extern "C" void cu_SHsphtor_to_spat_cplx_batch(shtns_cfg c, cplx *slm, cplx *tlm
                                                long spec_dist, long spat_dist) {
     clear_error();
     if (!cplx_batch_ok(c, "cu_SHsphtor_to_spat_cplx", nf, spec_dist, spat_dist)) return;
-    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = 2 * sizeof(double) * c->nspat * nf;
+    const bool direct = synth_pairs_ok(c);
+    const size_t qb = 2 * sizeof(double2) * c->nlm * nf, vb = direct ? 0 : 2 * sizeof(double) * c->nspat * nf;
     char *base = (char *)xscratch(c, 2 * qb + 2 * vb);
     if (!base) return;
     double2 *SS = (double2 *)base, *TT = (double2 *)(base + qb);
-    double *VT = (double *)(base + 2 * qb), *VP = (double *)(base + 2 * qb + vb);
     const hipStream_t s = c->comp_stream;
     launch_cplx_split(reinterpret_cast<const double2 *>(slm), spec_dist, SS, c->nlm, c->lmax, c->mmax, nf, s);
     launch_cplx_split(reinterpret_cast<const double2 *>(tlm), spec_dist, TT, c->nlm, c->lmax, c->mmax, nf, s);
-    vsynth_core(c, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), VT, VP, c->lmax, 2 * nf, c->nlm,
-                c->nspat);
-    if (g_last_err) return;
-    launch_interleave(VT, reinterpret_cast<double2 *>(zt), spat_dist, c->nspat, nf, s);
-    launch_interleave(VP, reinterpret_cast<double2 *>(zp), spat_dist, c->nspat, nf, s);
+    if (direct) {
+        const SpatPairs pt{reinterpret_cast<double *>(zt), spat_dist}, pp{reinterpret_cast<double *>(zp), spat_dist};
+        vsynth_core(c, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), nullptr, nullptr, c->lmax, 2 * nf,
+                    c->nlm, c->nspat, &pt, &pp);
+    } else {
+        double *VT = (double *)(base + 2 * qb), *VP = (double *)(base + 2 * qb + vb);
+        vsynth_core(c, reinterpret_cast<cplx *>(SS), reinterpret_cast<cplx *>(TT), VT, VP, c->lmax, 2 * nf, c->nlm,
+                    c->nspat);
+        if (g_last_err) return;
+        launch_interleave(VT, reinterpret_cast<double2 *>(zt), spat_dist, c->nspat, nf, s);
+        launch_interleave(VP, reinterpret_cast<double2 *>(zp), spat_dist, c->nspat, nf, s);
+    }
     HIP_OK(hipGetLastError());
 }
 extern "C" void cu_spat_cplx_to_SHsphtor_batch(shtns_cfg c, cplx *zt, cplx *zp, cplx *slm, cplx *tlm, int nf,

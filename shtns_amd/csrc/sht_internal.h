@@ -204,6 +204,16 @@ void launch_fft_synth(const DeviceTables *t, int nlat, int nphi, int mmax, int i
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields,
                      const double *V, long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s,
                      int lat_base = 0, int nlat_chunk = -1, int kind = 0, int es = 1);
+// Synthesis FFTs straight into interleaved complex fields: complex field y's parts p = 0, 1
+// from F + y fdist + p fpart into the doubles V + y spat_dist + p (sp, st: the real layout's
+// strides).  Only the cooperative split-exchange kernels do this; false (nothing launched)
+// when they are not the choice for this size and layout.  dry: only report that.
+bool launch_fft_synth_pairs(const DeviceTables *t, int nlat, int nphi, int imlim, int nf, const double2 *F, long fdist,
+                            long fpart, double *V, long spat_dist, long sp, long st, hipStream_t s, int kind = 0,
+                            bool dry = false);
+bool launch_fft_synth_coops_pairs(const DeviceTables *t, int nlat, int M, int imlim, int nf, const double2 *F,
+                                  long fdist, long fpart, double *V, long spat_dist, long sp, long st, hipStream_t s,
+                                  int nthreads, bool dry);
 // Register-resident FFT kernels (sht_fft.hip) for M = nphi/2 in {16, 24} x 2^k up to 6144,
 // latitudes [lat_base, lat_end); return false (nothing launched) for other M.
 bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,
