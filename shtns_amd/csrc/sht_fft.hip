@@ -733,11 +733,20 @@ template <int M, int NTB>
 __global__ __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(NTB == 256 ? 3 : 1))) void fft_anal_coops(const double *__restrict__ V, long spat_dist,
                                                       double2 *__restrict__ F, long four_dist,
                                                       const double2 *__restrict__ roots, int nlat, int mmax,
-                                                      int imlim, long sp, long stt, int lat_base, int lat_end) {
+                                                      int imlim, long sp, long stt, int lat_base, int lat_end,
+                                                      long fpart) {
     using CG = CoopSGeom<M, NTB>;
     constexpr int R = CG::R, LQ = CG::LQ, T = CG::T, STR = CG::STR, ROWS = CG::ROWS;
     extern __shared__ double ldsd[];
-    const int lat0 = lat_base + xcd_group_w(blockIdx.x, gridDim.x) * T;
+    // fpart > 0: the two parts of interleaved complex fields, as in fft_synth_coops (part =
+    // block-index bit 3: the blocks reading the two halves of a line run on one XCD together)
+    int bx = blockIdx.x, nbx = gridDim.x, part = 0;
+    if (fpart > 0) {
+        part = (bx >> 3) & 1;
+        bx = ((bx >> 4) << 3) | (bx & 7);
+        nbx >>= 1;
+    }
+    const int lat0 = lat_base + xcd_group_w(bx, nbx) * T;
     const int tl = threadIdx.x % T, row = threadIdx.x / T;
     const bool live = lat0 + tl < lat_end;
     double *mine = ldsd + tl * STR;
@@ -746,7 +755,7 @@ __global__ __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(NTB == 256 
     // z_n = V[2n] + i V[2n+1] of the block's latitudes, latitude fastest
     double zx[R], zy[R];
     {
-        const double *Vf = V + (size_t)blockIdx.y * spat_dist;
+        const double *Vf = V + (size_t)blockIdx.y * spat_dist + part;
         const unsigned off = 8u * (unsigned)((live ? lat0 + tl : 0) * stt + 2 * row * sp);
         const unsigned step = 8u * (unsigned)(2 * ROWS * sp), odd = 8u * (unsigned)sp;
         sfor<R>([&](auto ic) {
@@ -817,7 +826,7 @@ __global__ __launch_bounds__(NTB) __attribute__((amdgpu_waves_per_eu(NTB == 256 
     });
     __syncthreads();
     if (live) {
-        double2 *Ff = F + (size_t)blockIdx.y * four_dist;
+        double2 *Ff = F + (size_t)blockIdx.y * four_dist + (size_t)part * fpart;
         const unsigned off = 16u * (unsigned)(lat0 + tl + row * nlat), step = 16u * (unsigned)(ROWS * nlat);
         sfor<R>([&](auto ic) {
             constexpr int i = decltype(ic)::value;
@@ -954,14 +963,18 @@ bool synth_coops(const DeviceTables *t, int nlat, int M, int imlim, int nfields,
 
 template <int NTB>
 bool anal_coops(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nfields, const double *V,
-                long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end) {
+                long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base, int lat_end,
+                long fpart = 0, bool dry = false) {
 #define ANA_COOPS(MM)                                                                                         \
     if (M == MM) {                                                                                            \
         using CG = CoopSGeom<MM, NTB>;                                                                        \
         lds_limit(fft_anal_coops<MM, NTB>, CG::lds_bytes);                                                    \
         const int nb = (lat_end - lat_base + CG::T - 1) / CG::T;                                             \
-        hipLaunchKernelGGL((fft_anal_coops<MM, NTB>), dim3(nb, nfields), dim3(CG::NT), CG::lds_bytes, s, V,   \
-                           spat_dist, F, four_dist, t->roots, nlat, mmax, imlim, sp, st, lat_base, lat_end);  \
+        if (fpart > 0 && nb % 8) return false;   /* the part / XCD decoding needs whole groups of 8 */        \
+        if (dry) return true;                                                                                 \
+        hipLaunchKernelGGL((fft_anal_coops<MM, NTB>), dim3(fpart > 0 ? 2 * nb : nb, nfields), dim3(CG::NT),   \
+                           CG::lds_bytes, s, V, spat_dist, F, four_dist, t->roots, nlat, mmax, imlim, sp, st, \
+                           lat_base, lat_end, fpart);                                                         \
         return true;                                                                                          \
     }
     COOP_FFT_SIZES(ANA_COOPS)
@@ -999,6 +1012,17 @@ bool launch_fft_synth_coops_pairs(const DeviceTables *t, int nlat, int M, int im
         return synth_coops<512>(t, nlat, M, imlim, nf, F, fdist, V, spat_dist, sp, st, s, 0, nlat, fpart, dry);
     if (nthreads == -256)
         return synth_coops<256>(t, nlat, M, imlim, nf, F, fdist, V, spat_dist, sp, st, s, 0, nlat, fpart, dry);
+    return false;
+}
+
+bool launch_fft_anal_coops_pairs(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nf, const double *V,
+                                 long spat_dist, double2 *F, long fdist, long fpart, long sp, long st, hipStream_t s,
+                                 int nthreads, bool dry) {
+    if (fpart <= 0 || !coop_offsets_fit(nlat, M, std::max(mmax, imlim), sp, st)) return false;
+    if (nthreads == -512)
+        return anal_coops<512>(t, nlat, M, mmax, imlim, nf, V, spat_dist, F, fdist, sp, st, s, 0, nlat, fpart, dry);
+    if (nthreads == -256)
+        return anal_coops<256>(t, nlat, M, mmax, imlim, nf, V, spat_dist, F, fdist, sp, st, s, 0, nlat, fpart, dry);
     return false;
 }
 

@@ -1416,9 +1416,13 @@ static void fft_anal_from(shtns_cfg c, int imlim, int nf, int k0, const double *
                         four_dist, sp, st, s, 0, -1, kind);
         return;
     }
+    const double *zb = zp->z + (size_t)(k0 / 2) * 2 * zp->zdist;
+    if (launch_fft_anal_pairs(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf / 2, zb, 2 * zp->zdist, F, 2 * four_dist,
+                              four_dist, sp, st, s, kind))
+        return;   // both parts in one launch
     for (int part = 0; part < 2; ++part)
-        launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf / 2, zp->z + (size_t)(k0 / 2) * 2 * zp->zdist + part,
-                        2 * zp->zdist, F + (size_t)part * four_dist, 2 * four_dist, sp, st, s, 0, -1, kind, 2);
+        launch_fft_anal(&c->d, c->nlat, c->nphi, c->mmax, imlim, nf / 2, zb + part, 2 * zp->zdist,
+                        F + (size_t)part * four_dist, 2 * four_dist, sp, st, s, 0, -1, kind, 2);
 }
 
 // Scratch bytes of one synth_core call (Fourier rows F, then the pre-mixed coefficients).

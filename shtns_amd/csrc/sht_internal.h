@@ -214,6 +214,14 @@ bool launch_fft_synth_pairs(const DeviceTables *t, int nlat, int nphi, int imlim
 bool launch_fft_synth_coops_pairs(const DeviceTables *t, int nlat, int M, int imlim, int nf, const double2 *F,
                                   long fdist, long fpart, double *V, long spat_dist, long sp, long st, hipStream_t s,
                                   int nthreads, bool dry);
+// The analysis counterpart: complex field y's parts p read from the doubles V + y spat_dist + p
+// into F + y fdist + p fpart, one launch.
+bool launch_fft_anal_pairs(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nf, const double *V,
+                           long spat_dist, double2 *F, long fdist, long fpart, long sp, long st, hipStream_t s,
+                           int kind = 0, bool dry = false);
+bool launch_fft_anal_coops_pairs(const DeviceTables *t, int nlat, int M, int mmax, int imlim, int nf, const double *V,
+                                 long spat_dist, double2 *F, long fdist, long fpart, long sp, long st, hipStream_t s,
+                                 int nthreads, bool dry);
 // Register-resident FFT kernels (sht_fft.hip) for M = nphi/2 in {16, 24} x 2^k up to 6144,
 // latitudes [lat_base, lat_end); return false (nothing launched) for other M.
 bool launch_fft_synth_wave(const DeviceTables *t, int nlat, int M, int imlim, int nfields, const double2 *F,

@@ -931,6 +931,17 @@ bool launch_fft_synth_pairs(const DeviceTables *t, int nlat, int nphi, int imlim
                                         coop_threads(choice), dry);
 }
 
+bool launch_fft_anal_pairs(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nf, const double *V,
+                           long spat_dist, double2 *F, long fdist, long fpart, long sp, long st, hipStream_t s, int kind,
+                           bool dry) {
+    const int M = nphi / 2;
+    if (getenv("SHTNS_AMD_FFT_GENERIC") || !fft_coop_supported(M)) return false;
+    const int choice = fft_choice(M, sp, kind);
+    if (choice != 5 && choice != 6) return false;
+    return launch_fft_anal_coops_pairs(t, nlat, M, mmax, imlim, nf, V, spat_dist, F, fdist, fpart, 2 * sp, 2 * st, s,
+                                       coop_threads(choice), dry);
+}
+
 void launch_fft_anal(const DeviceTables *t, int nlat, int nphi, int mmax, int imlim, int nfields, const double *V,
                      long spat_dist, double2 *F, long four_dist, long sp, long st, hipStream_t s, int lat_base,
                      int nlat_chunk, int kind, int es) {
