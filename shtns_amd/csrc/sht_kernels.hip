@@ -286,11 +286,22 @@ __device__ __forceinline__ long lm_cplx(int l, int m, int mmax) {
     return (l <= mmax) ? (long)l * (l + 1) + m : (long)mmax * (2 * l - mmax) + l + m;
 }
 
+// Block b -> order m: groups of 8 consecutive orders on one XCD (blocks are dealt to the 8
+// XCDs round-robin; XCD x runs blocks x, x + 8, ...), the groups dealt round-robin to the
+// XCDs so that the triangular work stays balanced.  The orders of a group read (write)
+// neighbouring elements of the same l-rows of the complex layout, which then meet in one L2.
+// Identity unless the count is a multiple of 64.
+__device__ __forceinline__ int order_of_block(int b, int nb) {
+    if (nb % 64) return b;
+    const int x = b & 7, j = b >> 3;
+    return 8 * (x + 8 * (j >> 3)) + (j & 7);
+}
+
 // Batches: field f of the complex spectra at a + f * a_dist; its two real spectra X_f, Y_f at
 // XY + 2 f nlm and XY + (2 f + 1) nlm (one batch of 2 nf real fields, field distance nlm).
 __global__ void cplx_split_kernel(const double2 *__restrict__ a, long a_dist, double2 *__restrict__ XY, long nlm,
                                   int lmax, int mmax) {
-    const int m = blockIdx.x;
+    const int m = order_of_block(blockIdx.x, gridDim.x);
     const double2 *af = a + (size_t)blockIdx.y * a_dist;
     double2 *X = XY + (size_t)blockIdx.y * 2 * nlm, *Y = X + nlm;
     const double sgn = (m & 1) ? -1.0 : 1.0;
@@ -311,7 +322,7 @@ __global__ void cplx_split_kernel(const double2 *__restrict__ a, long a_dist, do
 
 __global__ void cplx_merge_kernel(const double2 *__restrict__ XY, long nlm, double2 *__restrict__ a, long a_dist,
                                   int lmax, int mmax) {
-    const int m = blockIdx.x;
+    const int m = order_of_block(blockIdx.x, gridDim.x);
     double2 *af = a + (size_t)blockIdx.y * a_dist;
     const double2 *X = XY + (size_t)blockIdx.y * 2 * nlm, *Y = X + nlm;
     const double sgn = (m & 1) ? -1.0 : 1.0;
