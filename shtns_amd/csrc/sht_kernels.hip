@@ -286,26 +286,21 @@ __device__ __forceinline__ long lm_cplx(int l, int m, int mmax) {
     return (l <= mmax) ? (long)l * (l + 1) + m : (long)mmax * (2 * l - mmax) + l + m;
 }
 
-// Block b -> order m: groups of 8 consecutive orders on one XCD (blocks are dealt to the 8
-// XCDs round-robin; XCD x runs blocks x, x + 8, ...), the groups dealt round-robin to the
-// XCDs so that the triangular work stays balanced.  The orders of a group read (write)
-// neighbouring elements of the same l-rows of the complex layout, which then meet in one L2.
-// Identity unless the count is a multiple of 64.
-__device__ __forceinline__ int order_of_block(int b, int nb) {
-    if (nb % 64) return b;
-    const int x = b & 7, j = b >> 3;
-    return 8 * (x + 8 * (j >> 3)) + (j & 7);
-}
-
 // Batches: field f of the complex spectra at a + f * a_dist; its two real spectra X_f, Y_f at
 // XY + 2 f nlm and XY + (2 f + 1) nlm (one batch of 2 nf real fields, field distance nlm).
+// A block takes 8 consecutive orders m0 .. m0+7 (lane i = thread % 8) and blockDim/8 degrees
+// at a time: for one l the 8 lanes touch 8 consecutive elements (128 bytes) of each l-row of
+// the l-major complex layout, and for one order the degrees run contiguously in the m-major
+// real layout, so both sides are coalesced.  No block-level synchronization.
 __global__ void cplx_split_kernel(const double2 *__restrict__ a, long a_dist, double2 *__restrict__ XY, long nlm,
                                   int lmax, int mmax) {
-    const int m = order_of_block(blockIdx.x, gridDim.x);
+    const int i = threadIdx.x & 7, m0 = 8 * blockIdx.x, m = m0 + i;
+    if (m > mmax) return;
     const double2 *af = a + (size_t)blockIdx.y * a_dist;
     double2 *X = XY + (size_t)blockIdx.y * 2 * nlm, *Y = X + nlm;
     const double sgn = (m & 1) ? -1.0 : 1.0;
-    for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
+    for (int l = m0 + (int)(threadIdx.x >> 3); l <= lmax; l += blockDim.x >> 3) {
+        if (l < m) continue;
         const double2 ap = af[lm_cplx(l, m, mmax)];
         const long lm = lm_real(l, m, lmax);
         if (m == 0) {
@@ -322,11 +317,13 @@ __global__ void cplx_split_kernel(const double2 *__restrict__ a, long a_dist, do
 
 __global__ void cplx_merge_kernel(const double2 *__restrict__ XY, long nlm, double2 *__restrict__ a, long a_dist,
                                   int lmax, int mmax) {
-    const int m = order_of_block(blockIdx.x, gridDim.x);
+    const int i = threadIdx.x & 7, m0 = 8 * blockIdx.x, m = m0 + i;
+    if (m > mmax) return;
     double2 *af = a + (size_t)blockIdx.y * a_dist;
     const double2 *X = XY + (size_t)blockIdx.y * 2 * nlm, *Y = X + nlm;
     const double sgn = (m & 1) ? -1.0 : 1.0;
-    for (int l = m + threadIdx.x; l <= lmax; l += blockDim.x) {
+    for (int l = m0 + (int)(threadIdx.x >> 3); l <= lmax; l += blockDim.x >> 3) {
+        if (l < m) continue;
         const long lm = lm_real(l, m, lmax);
         const double2 x = X[lm], y = Y[lm];
         af[lm_cplx(l, m, mmax)] = make_double2(x.x - y.y, x.y + y.x);
@@ -346,11 +343,11 @@ __global__ void interleave_kernel(const double *__restrict__ xy, double2 *__rest
 
 void launch_cplx_split(const double2 *a, long a_dist, double2 *XY, long nlm, int lmax, int mmax, int nf, hipStream_t s) {
     if (nf <= 0) return;
-    hipLaunchKernelGGL(cplx_split_kernel, dim3(mmax + 1, nf), dim3(256), 0, s, a, a_dist, XY, nlm, lmax, mmax);
+    hipLaunchKernelGGL(cplx_split_kernel, dim3((mmax + 8) / 8, nf), dim3(256), 0, s, a, a_dist, XY, nlm, lmax, mmax);
 }
 void launch_cplx_merge(const double2 *XY, long nlm, double2 *a, long a_dist, int lmax, int mmax, int nf, hipStream_t s) {
     if (nf <= 0) return;
-    hipLaunchKernelGGL(cplx_merge_kernel, dim3(mmax + 1, nf), dim3(256), 0, s, XY, nlm, a, a_dist, lmax, mmax);
+    hipLaunchKernelGGL(cplx_merge_kernel, dim3((mmax + 8) / 8, nf), dim3(256), 0, s, XY, nlm, a, a_dist, lmax, mmax);
 }
 void launch_interleave(const double *xy, double2 *z, long z_dist, long n, int nf, hipStream_t s) {
     if (n <= 0 || nf <= 0) return;
