@@ -72,6 +72,14 @@ __device__ __forceinline__ double quad_perm(double v) {
     return __hiloint2double(hi, lo);
 }
 
+// v of lane l - 1 within each row of 16 lanes (DPP row_shr:1); lanes 0, 16, 32, 48 get 0.
+// All lanes of the wave must be active.
+__device__ __forceinline__ double row_shr1(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x111, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x111, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+
 // ---------------------------------------------------------------------------------------
 // Contraction precision PREC (the recurrence, seeds, tables and the final sums stay FP64):
 //   0  FP64 matrix cores: v_mfma_f64_16x16x4f64 on z and the coefficients;
@@ -527,36 +535,78 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
         // load issued before the first use: the block's first MFMA waits for these.
         const int col = NF4 ? (lane & 3) : (lane & 15), comp = col & 3;
         const bool keep = nsteps > 0 && !(o.real_only && (comp & 1));
+        const int kl0 = (tile_base + tw + NW * r) * 64 + (lane >> 4);   // latitude of K-group 0
+        if constexpr (NLR >= 3) {
 #pragma unroll
-        for (int c = 0; c < NA; ++c) {
-            const int f = NF4 ? fbase + c : fbase + 4 * c + (col >> 2);
-            const bool fv = keep && f < nfields;
-            const double2 *Ff = F + (size_t)(fv ? f : 0) * four_dist + (size_t)im * nlat;
+            for (int c = 0; c < NA; ++c) {
+                const int f = NF4 ? fbase + c : fbase + 4 * c + (col >> 2);
+                const bool fv = keep && f < nfields;
+                const double2 *Ff = F + (size_t)(fv ? f : 0) * four_dist + (size_t)im * nlat;
 #pragma unroll
-            for (int kg = 0; kg < 16; ++kg) {
-                const int kl = (tile_base + tw + NW * r) * 64 + 4 * kg + (lane >> 4);
-                const bool kv = fv && kl < nlat_2;
-                if constexpr (NLR >= 3) {
+                for (int kg = 0; kg < 16; ++kg) {
                     // several tiles per wave: one K-group at a time (loads hoisted together
                     // would need more registers than the tiles' operands leave)
+                    const int kl = kl0 + 4 * kg;
                     double v = 0.0;
-                    if (kv) {
+                    if (fv && kl < nlat_2) {
                         const double2 fn = Ff[kl], fs = Ff[nlat - 1 - kl];
                         const double w = lat_weight ? wqk[kl] : ((2 * kl + 1 == nlat) ? 0.5 : 1.0);   // odd grid: equator row once
                         v = (comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
                              : comp == 2 ? (fn.x - fs.x) * xk[kl] : (fn.y - fs.y) * xk[kl]) * w;
                     }
                     Ereg[r][c][kg] = C::split(v);
-                } else {
-                    const int klc = kv ? kl : 0;
-                    const double2 fn = Ff[klc];
-                    const double2 fs = Ff[nlat - 1 - klc];
-                    const double w = lat_weight ? wqk[klc] : ((2 * klc + 1 == nlat) ? 0.5 : 1.0);
-                    const double x = xk[klc];
-                    const double v = comp == 0 ? fn.x + fs.x : comp == 1 ? fn.y + fs.y
-                                   : comp == 2 ? (fn.x - fs.x) * x : (fn.y - fs.y) * x;
-                    Ereg[r][c][kg] = C::split(kv ? v * w : 0.0);
                 }
+            }
+        } else {
+            // Batches of KB K-groups: every load of a batch is issued before its first use
+            // (clamped indices, no branches; each lane loads only the real or imaginary part
+            // its column takes), and the batches are fenced so the scheduler does not hoist
+            // later batches' loads above them.  The arithmetic is that of the branchy form
+            // above, rounding for rounding: (a + b) w for E, ((a - b) x) w for O, with a - b
+            // as one fma by -1.
+            constexpr int KB = 4;
+            const bool neg = comp & 2;
+            const double sgn = neg ? -1.0 : 1.0;
+            const double *Fc[NA];
+            bool fvc[NA];
+#pragma unroll
+            for (int c = 0; c < NA; ++c) {
+                const int f = NF4 ? fbase + c : fbase + 4 * c + (col >> 2);
+                fvc[c] = keep && f < nfields;
+                Fc[c] = reinterpret_cast<const double *>(F + (size_t)(fvc[c] ? f : 0) * four_dist + (size_t)im * nlat) +
+                        (comp & 1);
+            }
+#pragma unroll
+            for (int kb = 0; kb < 16; kb += KB) {
+                double xq[KB], wq[KB], aq[NA][KB], bq[NA][KB];
+#pragma unroll
+                for (int q = 0; q < KB; ++q) {
+                    const int kl = kl0 + 4 * (kb + q);
+                    const int klx = (kl < nlat_2) ? kl : 0;
+                    xq[q] = xk[klx];
+                    const double wl = wqk[klx];
+                    wq[q] = lat_weight ? wl : ((2 * klx + 1 == nlat) ? 0.5 : 1.0);   // odd grid: equator row once
+#pragma unroll
+                    for (int c = 0; c < NA; ++c) {
+                        const int klc = (fvc[c] && kl < nlat_2) ? kl : 0;
+                        aq[c][q] = Fc[c][2 * klc];
+                        bq[c][q] = Fc[c][2 * (nlat - 1 - klc)];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < KB; ++q) {
+                    const int kl = kl0 + 4 * (kb + q);
+#pragma unroll
+                    for (int c = 0; c < NA; ++c) {
+                        const double s = fma(sgn, bq[c][q], aq[c][q]);
+                        double v = (fvc[c] && kl < nlat_2) ? (neg ? s * xq[q] : s) * wq[q] : 0.0;
+                        // computed here, not sunk towards the MFMAs (which would keep every
+                        // batch's loaded values live across the prologue)
+                        asm volatile("" : "+v"(v));
+                        Ereg[r][c][kb + q] = C::split(v);
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
             }
         }
     }
@@ -591,8 +641,12 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     double2 *Qbase = Q + o.spec_base;
 
     // post-mix work split: thread handles elements e = threadIdx.x + j*NT (j < NE) of the
-    // 16 x NCOL block, each for all NG field groups (the same coefficient row); the
-    // coefficient rows are prefetched before the MFMA phase.
+    // 16 x NCOL block, step row e % 16 of column e / 16, each for all NG field groups (the
+    // same coefficient row); the coefficient rows are prefetched before the MFMA phase.
+    // Each 16-lane DPP row holds the 16 steps of one column, so the sum of step row - 1 (the
+    // even degrees' second term) is the neighbouring lane's, and a wave's stores cover the
+    // 32 consecutive degrees of one field (16 NCOL is a multiple of 64: whole waves take part
+    // or skip an element together).
     constexpr int NE = (16 * NCOL + NT - 1) / NT;
     auto cf_at = [&](auto from_lds, int idx) {
         idx = min(idx, nsteps - 1);
@@ -635,7 +689,7 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             const int e = threadIdx.x + j * NT;
-            const int row = min(e / NCOL, 15);
+            const int row = e & 15;
             const int ii = min(i + row, nsteps - 1);
             const double4 cf4 = pmm[ii];
             pcx[j] = cf4.x;
@@ -693,28 +747,26 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             const int e = threadIdx.x + j * NT;
-            if (e >= 16 * NCOL) break;
-            const int row = e / NCOL, col = e - row * NCOL;
+            if (e >= 16 * NCOL) break;   // wave-uniform
+            const int row = e & 15, col = e >> 4;
             const int comp = col & 3;
             const int ii = i + row;
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
                 const int f = fblk + g * 4 * NCG + (col >> 2);
-                double sum = 0.0, sprev = 0.0;
+                double sum = 0.0;
 #pragma unroll
                 for (int w = 0; w < NW; ++w) sum += red[g][w][row][col];
+                // step row - 1 of this column: lane - 1 (the whole wave is active here), or
+                // the previous group's last row
+                double sprev = row_shr1(sum);
+                if (row == 0) sprev = carry[g][pb ^ 1][col];
                 if (row == 15 && comp < 2) carry[g][pb][col] = sum;
                 if (f >= nfields || ii >= nsteps_out) continue;
                 const int le = o.mfun + 2 * ii;
                 double v = 0.0;
                 if (ii < nsteps) {
                     if (comp < 2) {
-                        if (row > 0) {
-#pragma unroll
-                            for (int w = 0; w < NW; ++w) sprev += red[g][w][row - 1][col];
-                        } else {
-                            sprev = carry[g][pb ^ 1][col];
-                        }
                         v = pcx[j] * sum + pcp[j] * sprev;
                     } else {
                         v = pcz[j] * sum;
