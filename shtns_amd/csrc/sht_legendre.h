@@ -471,7 +471,9 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     // LDS once per block (dynamic LDS), so each 16-step group reads them at LDS latency instead
     // of waiting on scalar loads that miss to HBM
     extern __shared__ double2 cfl[];
-    __shared__ double red[NG][NW][16][NCOL + 1];   // single-buffered: two barriers per block
+    // single-buffered: two barriers per block.  Rows padded to NCOL + 2 doubles: a post-mix read
+    // (32 lanes = steps 0..15 of two adjacent columns) then falls on 32 distinct bank pairs
+    __shared__ double red[NG][NW][16][NCOL + 2];
     __shared__ double carry[NG][2][NCOL];
     __shared__ int s_ibeg[NW * NG];
 
@@ -804,7 +806,7 @@ void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, doub
     // static LDS of the kernel, and the coefficient table if it fits beside it in 160 KB
     constexpr size_t NCOL = NF4 ? 4 * NF4 : 16 * NCG;
     constexpr size_t static_lds =
-        sizeof(double) * NG * (NW * 16 * 65 + NW * 16 * (NCOL + 1) + 2 * NCOL) + sizeof(int) * NW * NG;
+        sizeof(double) * NG * (NW * 16 * 65 + NW * 16 * (NCOL + 2) + 2 * NCOL) + sizeof(int) * NW * NG;
     const size_t cf_bytes = sizeof(double2) * (size_t)(a.ltr / 2 + 2);
     const int cf_lds = (static_lds + cf_bytes + 1024 <= 160 * 1024) ? 1 : 0;
     const size_t dyn = cf_lds ? cf_bytes : 0;
