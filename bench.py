@@ -47,7 +47,7 @@ def fft_flops(nlat, nphi):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "sht_pair_l1023_b16_hbm_traffic.txt")
 # rows of the FP64 kernels of this workload in that summary
 TRAFFIC_ROWS = {"legendre_synth": ("premix_kernel<64>", "legendre_synth_kernel<4, 0, 0>"),
-                "legendre_anal": ("legendre_anal_kernel<12, 1, 2, 0, 0, 1>",), "fft_synth": ("fft_synth",),
+                "legendre_anal": ("legendre_anal_kernel<12, 1, 2, 0, 0, 1, 1>",), "fft_synth": ("fft_synth",),
                 "fft_anal": ("fft_anal",)}
 
 

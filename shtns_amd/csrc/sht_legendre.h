@@ -453,7 +453,10 @@ void synth_tiles_launch(const LegendreArgs &a, int spec_ltop, double2 *F, long f
 // NG > 1 (one tile per wave): the block holds NG field groups of the same order, waves
 // w = tw + NW g (tile tw, group g); each group's NW partial tiles are reduced separately.
 // Six-tile grids then run 12-wave blocks (three waves on every SIMD) instead of 6-wave ones.
-template <int NW, int NLR, int NCG, int PREC, int NF4 = 0, int NG = 1>
+// GR > 1 (4x4x4 path, one tile per wave): GR 16-step groups share one barrier pair and one
+// reduction round (16 GR rows of partial tiles), for batches whose MFMA work per group is
+// too small to cover those fixed costs.
+template <int NW, int NLR, int NCG, int PREC, int NF4 = 0, int NG = 1, int GR = 1>
 __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     const double *__restrict__ xk, const double *__restrict__ x2k, const double *__restrict__ stk,
     const double *__restrict__ wqk, const int *__restrict__ step_ofs, const double2 *__restrict__ ab,
@@ -463,6 +466,8 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     double mpos_scale, int lat_weight, int cf_lds, int tile_base) {
     static_assert(NF4 == 0 || (PREC == 0 && NCG == 1 && NF4 <= 3), "4x4x4 path: FP64, one group of <= 3 fields");
     static_assert(NG == 1 || (NF4 == 0 && NLR == 1), "several field groups per block: one tile per wave");
+    static_assert(GR == 1 || (GR == 2 && NF4 > 0 && NLR == 1 && NG == 1), "two groups per round: 4x4x4 path, one tile");
+    constexpr int RR = 16 * GR;                   // steps per reduction round
     constexpr int NA = NF4 ? NF4 : NCG;           // operand / accumulator sets
     constexpr int NCOL = NF4 ? 4 * NF4 : 16 * NCG;
     constexpr int NT = 64 * NW * NG;              // threads per block
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     extern __shared__ double2 cfl[];
     // single-buffered: two barriers per block.  Rows padded to NCOL + 2 doubles: a post-mix read
     // (32 lanes = steps 0..15 of two adjacent columns) then falls on 32 distinct bank pairs
-    __shared__ double red[NG][NW][16][NCOL + 2];
+    __shared__ double red[NG][NW][RR][NCOL + 2];
     __shared__ double carry[NG][2][NCOL];
     __shared__ int s_ibeg[NW * NG];
 
@@ -649,7 +654,7 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     // even degrees' second term) is the neighbouring lane's, and a wave's stores cover the
     // 32 consecutive degrees of one field (16 NCOL is a multiple of 64: whole waves take part
     // or skip an element together).
-    constexpr int NE = (16 * NCOL + NT - 1) / NT;
+    constexpr int NE = (RR * NCOL + NT - 1) / NT;
     auto cf_at = [&](auto from_lds, int idx) {
         idx = min(idx, nsteps - 1);
         if constexpr (decltype(from_lds)::value) return cfl[idx];   // uniform: LDS broadcast
@@ -686,12 +691,12 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
         }
     };
     int b = 0;
-    for (int i = ibeg; i < nsteps; i += 16, ++b) {
+    for (int i = ibeg; i < nsteps; i += RR, ++b) {
         double pcx[NE], pcz[NE], pcp[NE];
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             const int e = threadIdx.x + j * NT;
-            const int row = e & 15;
+            const int row = e & (RR - 1);
             const int ii = min(i + row, nsteps - 1);
             const double4 cf4 = pmm[ii];
             pcx[j] = cf4.x;
@@ -699,15 +704,18 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
             pcp[j] = (ii > 0) ? pmm[ii - 1].y : 0.0;
         }
         using accA_t = std::conditional_t<NF4 == 0, typename C::acc_t, double>;
-        accA_t acc[NA], acc2[NA];   // two chains per column group (even / odd K-groups)
+        accA_t acc[GR][NA], acc2[GR][NA];   // two chains per column group (even / odd K-groups)
 #pragma unroll
-        for (int c = 0; c < NA; ++c) acc[c] = acc2[c] = accA_t{};
-        {
+        for (int q = 0; q < GR; ++q)
+#pragma unroll
+            for (int c = 0; c < NA; ++c) acc[q][c] = acc2[q][c] = accA_t{};
+#pragma unroll
+        for (int q = 0; q < GR; ++q) {
             double2 cf[16];
-            load_cf16(cf, i);
+            load_cf16(cf, i + 16 * q);
 #pragma unroll
             for (int r = 0; r < NLR; ++r) {
-                recur_plain(r, i, cf);
+                recur_plain(r, i + 16 * q, cf);
                 typename C::a_t a[16];
                 __builtin_amdgcn_s_waitcnt(0xc07f);
                 __builtin_amdgcn_wave_barrier();
@@ -720,37 +728,42 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
 #pragma unroll
                     for (int c = 0; c < NA; ++c) {
                         if constexpr (NF4 > 0) {
-                            acc[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[kg], Ereg[r][c][kg], acc[c], 0, 0, 0);
-                            acc2[c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[kg + 1], Ereg[r][c][kg + 1], acc2[c], 0, 0, 0);
+                            acc[q][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[kg], Ereg[r][c][kg], acc[q][c], 0, 0, 0);
+                            acc2[q][c] = __builtin_amdgcn_mfma_f64_4x4x4f64(a[kg + 1], Ereg[r][c][kg + 1], acc2[q][c], 0, 0, 0);
                         } else {
-                            acc[c] = C::mma(a[kg], Ereg[r][c][kg], acc[c]);
-                            acc2[c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[c]);
+                            acc[q][c] = C::mma(a[kg], Ereg[r][c][kg], acc[q][c]);
+                            acc2[q][c] = C::mma(a[kg + 1], Ereg[r][c][kg + 1], acc2[q][c]);
                         }
                     }
             }
         }
 #pragma unroll
-        for (int c = 0; c < NA; ++c) acc[c] += acc2[c];
+        for (int q = 0; q < GR; ++q)
+#pragma unroll
+            for (int c = 0; c < NA; ++c) acc[q][c] += acc2[q][c];
         const int pb = b & 1;
         __syncthreads();   // previous block's post-mix has finished reading red
         if constexpr (NF4 > 0) {
             // 4x4x4 D: step 4 ((lane >> 2) & 3) + (lane >> 4), column lane & 3 of field c
 #pragma unroll
-            for (int c = 0; c < NA; ++c) red[gw][tw][4 * ((lane >> 2) & 3) + (lane >> 4)][4 * c + (lane & 3)] = acc[c];
+            for (int q = 0; q < GR; ++q)
+#pragma unroll
+                for (int c = 0; c < NA; ++c)
+                    red[gw][tw][16 * q + 4 * ((lane >> 2) & 3) + (lane >> 4)][4 * c + (lane & 3)] = acc[q][c];
         } else {
             // D layout (gfx950): acc[c][j] = D[step row 4*j + lane/16][col lane%16]
 #pragma unroll
             for (int c = 0; c < NCG; ++c)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) red[gw][tw][C::d_row(j, lane)][16 * c + (lane & 15)] = acc[c][j];
+                for (int j = 0; j < 4; ++j) red[gw][tw][C::d_row(j, lane)][16 * c + (lane & 15)] = acc[0][c][j];
         }
         __syncthreads();
         // sum over each group's waves + post-mix
 #pragma unroll
         for (int j = 0; j < NE; ++j) {
             const int e = threadIdx.x + j * NT;
-            if (e >= 16 * NCOL) break;   // wave-uniform
-            const int row = e & 15, col = e >> 4;
+            if (e >= RR * NCOL) break;   // wave-uniform
+            const int row = e & (RR - 1), col = e / RR;
             const int comp = col & 3;
             const int ii = i + row;
 #pragma unroll
@@ -759,11 +772,12 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
                 double sum = 0.0;
 #pragma unroll
                 for (int w = 0; w < NW; ++w) sum += red[g][w][row][col];
-                // step row - 1 of this column: lane - 1 (the whole wave is active here), or
-                // the previous group's last row
-                double sprev = row_shr1(sum);
+                // step row - 1 of this column: lane - 1 (the whole wave is active here; within
+                // a DPP row for 16-row rounds, a wave-wide shift for 32), or the previous
+                // round's last row
+                double sprev = (GR == 1) ? row_shr1(sum) : __shfl_up(sum, 1);
                 if (row == 0) sprev = carry[g][pb ^ 1][col];
-                if (row == 15 && comp < 2) carry[g][pb][col] = sum;
+                if (row == RR - 1 && comp < 2) carry[g][pb][col] = sum;
                 if (f >= nfields || ii >= nsteps_out) continue;
                 const int le = o.mfun + 2 * ii;
                 double v = 0.0;
@@ -785,7 +799,7 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     }
     // degrees not reached by the step loop: before ibeg (negligible functions) and after nsteps
     for (int ii = threadIdx.x; ii < nsteps_out; ii += NT) {
-        if (ii >= ibeg && ii < ibeg + ((nsteps - ibeg + 15) & ~15)) continue;
+        if (ii >= ibeg && ii < ibeg + ((nsteps - ibeg + RR - 1) / RR) * RR) continue;
         const int le = o.mfun + 2 * ii;
         for (int f = fblk; f < min(nfields, fblk + 4 * NCG * NG); ++f) {
             double2 *Qf = Qbase + (size_t)f * spec_dist;
@@ -798,20 +812,28 @@ __global__ __launch_bounds__(64 * NW * NG) void legendre_anal_kernel(
     }
 }
 
-template <int NW, int NLR, int NCG, int PREC, int NF4 = 0, int NG = 1>
+// static LDS of legendre_anal_kernel, and whether the order's coefficient table (ltr/2 + 2
+// pairs) fits beside it in the 160 KB of a CU
+constexpr size_t anal_static_lds(int NW, int NG, int NCOL, int GR) {
+    return sizeof(double) * NG * (NW * 16 * 65 + NW * 16 * GR * (NCOL + 2) + 2 * NCOL) + sizeof(int) * NW * NG;
+}
+inline bool anal_cf_fits(size_t static_lds, int ltr) {
+    return static_lds + sizeof(double2) * (size_t)(ltr / 2 + 2) + 1024 <= 160 * 1024;
+}
+
+template <int NW, int NLR, int NCG, int PREC, int NF4 = 0, int NG = 1, int GR = 1>
 void launch_anal_t(const LegendreArgs &a, const double2 *F, long four_dist, double2 *Q, long spec_dist,
                  int spec_ltop, hipStream_t s, int tile_base = 0) {
     const DeviceTables *t = a.t;
     dim3 grid((a.mmax + 1) * ((a.nfields + 4 * NCG * NG - 1) / (4 * NCG * NG)));
     // static LDS of the kernel, and the coefficient table if it fits beside it in 160 KB
-    constexpr size_t NCOL = NF4 ? 4 * NF4 : 16 * NCG;
-    constexpr size_t static_lds =
-        sizeof(double) * NG * (NW * 16 * 65 + NW * 16 * (NCOL + 2) + 2 * NCOL) + sizeof(int) * NW * NG;
+    constexpr int NCOL = NF4 ? 4 * NF4 : 16 * NCG;
+    constexpr size_t static_lds = anal_static_lds(NW, NG, NCOL, GR);
     const size_t cf_bytes = sizeof(double2) * (size_t)(a.ltr / 2 + 2);
-    const int cf_lds = (static_lds + cf_bytes + 1024 <= 160 * 1024) ? 1 : 0;
+    const int cf_lds = anal_cf_fits(static_lds, a.ltr) ? 1 : 0;
     const size_t dyn = cf_lds ? cf_bytes : 0;
-    ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC, NF4, NG>, dyn);
-    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC, NF4, NG>), grid, dim3(64 * NW * NG), dyn, s, t->x, t->x2, t->st,
+    ensure_lds_limit((const void *)legendre_anal_kernel<NW, NLR, NCG, PREC, NF4, NG, GR>, dyn);
+    hipLaunchKernelGGL((legendre_anal_kernel<NW, NLR, NCG, PREC, NF4, NG, GR>), grid, dim3(64 * NW * NG), dyn, s, t->x, t->x2, t->st,
                        t->wq, t->step_ofs, t->ab, t->pm, a.vector_mode ? t->i0v : t->i0,
                        a.vector_mode ? t->z0v : t->z0, F, four_dist, Q, spec_dist,
                        spec_ltop == a.lmax + 1 ? a.lmax + 1 : a.lmax, spec_ltop, a.mmax, a.mres, a.nlat, a.nlat_2,
@@ -831,7 +853,11 @@ void anal_launch(const LegendreArgs &a, const double2 *F, long four_dist, double
         if (ns > 1 && a.part) {   // 12-wave blocks over a latitude split each, then the sum
             for (int k = 0; k < ns; ++k) {
                 double2 *P = a.part + (size_t)k * a.nfields * a.part_dist;
-                if (a.nfields == 1) launch_anal_t<12, 1, 1, 0, 1>(a, F, four_dist, P, a.part_dist, spec_ltop, s, 12 * k);
+                // one field: two 16-step groups per barrier round where the coefficient table
+                // still fits in LDS beside the larger partial-tile buffer
+                if (a.nfields == 1 && anal_cf_fits(anal_static_lds(12, 1, 4, 2), a.ltr))
+                    launch_anal_t<12, 1, 1, 0, 1, 1, 2>(a, F, four_dist, P, a.part_dist, spec_ltop, s, 12 * k);
+                else if (a.nfields == 1) launch_anal_t<12, 1, 1, 0, 1>(a, F, four_dist, P, a.part_dist, spec_ltop, s, 12 * k);
                 else launch_anal_t<12, 1, 1, 0, 2>(a, F, four_dist, P, a.part_dist, spec_ltop, s, 12 * k);
             }
             launch_sum_splits(a.part, a.part_dist, ns, a.nfields, Q, spec_dist, a.part_dist, s);
